@@ -1,0 +1,198 @@
+"""Benchmark: fwd + 1st + 2nd-order (Eikonal) steps of knn_aggregate_aniso on one GPU.
+
+One step = ``f_out = knn_aggregate_aniso(q, p, f, sigma, R, K)``; ``dq = d(f_out . v)/dq``
+with ``create_graph=True``; ``L = sum_q (||dq||_2 - 1)^2``; gradients of ``L`` with respect
+to q, p, f, sigma and R.  Default workload: one clustered cloud, Q = 2^20 queries,
+P = 2^16 points, K = 16, F = 64, float32, synthetic seeded data, inputs resident in HBM.
+
+Prints one JSON line: queries/s over the timed steps, the roofline of the dominant
+kernel (per-launch hipEvent times recorded by the library's tracer over the timed
+region), the HBM roofline of the gather/scatter kernels, and the plain-PyTorch CPU
+formulation timed on a bounded query sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "cuda-knn-aniso-gaussian-feature-aggregation_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+KEY_FLOP = {"aniso": 26, "euclidean": 8}
+METRIC = "fwd+bwd+2nd-order queries/s (Q=1M,P=64K,K=16,F=64) vs plain; served p50/p99 ms"
+
+WORKLOADS = {
+    "clustered": "clustered B=1 Q=1048576 P=65536 K=16 F=64",
+    "bench_small": "uniform B=2 Q=12288 P=4096 K=16 F=64",
+    "readme": "uniform B=2 Q=1000 P=500 K=12 F=32",
+}
+
+
+def make_batch(name, seed=0):
+    from cuda_knn_aggregate import workloads
+
+    return {"clustered": workloads.clustered, "bench_small": workloads.bench_small,
+            "readme": workloads.readme}[name](seed)
+
+
+def algorithmic_bytes(kernel, B, Q, P, F, K, el=4):
+    """Compulsory HBM bytes of one launch: inputs read once, outputs written once.
+    Point inputs p, sigma, R (15 values) and f (F values); int64 idxs."""
+    rows_q, rows_p = B * Q, B * P
+    pts_in = rows_p * (15 + F) * el
+    pts_out = rows_p * (15 + F) * el
+    if kernel == "aggregate":             # idxs, dists in; f_out, w_out out; f, q in
+        return rows_q * (3 * el + 8 * K + el * K + F * el + el * K) + rows_p * F * el
+    if kernel == "query_grad":            # g, q, idxs in; dq out; point inputs
+        return rows_q * (F * el + 3 * el + 8 * K + 3 * el) + pts_in
+    if kernel == "query_grad2":           # h, g, q, idxs in; dq out
+        return rows_q * (3 * el + F * el + 3 * el + 8 * K + 3 * el) + pts_in
+    if kernel == "point_grad":            # csr, q, g in; point grads out
+        return rows_q * K * 4 + (rows_p + 1) * 4 + rows_q * (3 * el + F * el) + pts_in + pts_out
+    if kernel == "point_grad2":
+        return rows_q * K * 4 + (rows_p + 1) * 4 + rows_q * (6 * el + F * el) + pts_in + pts_out
+    if kernel == "build_csr":             # idxs in, csr out
+        return rows_q * K * (8 + 4) + (rows_p + 1) * 4
+    if kernel == "pack_points":
+        return rows_p * (15 * el + 16 * el)
+    if kernel == "knn_topk":              # q, records in; idxs, dists out
+        return rows_q * (3 * el + 8 * K + el * K) + rows_p * 16 * el
+    return 0
+
+
+def cpu_baseline(name, budget_s=15.0):
+    """Plain-PyTorch CPU formulation (oracle.plain.eikonal_step, float32) on a query
+    sample of the same workload against all its points."""
+    import torch
+
+    from oracle import plain
+
+    torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
+    b = make_batch(name)
+    Qtot = b.q.shape[1]
+
+    def run(nq):
+        rows = torch.arange(0, Qtot, max(1, Qtot // nq))[:nq]
+        q = b.q[:, rows].contiguous()
+        t0 = time.perf_counter()
+        plain.eikonal_step(q, b.p, b.f, b.sigma, b.R, b.K, b.v)
+        return time.perf_counter() - t0, q.shape[0] * q.shape[1]
+
+    nq = min(256, Qtot)
+    t, n = run(nq)                                 # warm-up + size probe
+    nq = int(min(Qtot, max(nq, nq * budget_s / max(t, 1e-3) / 2)))
+    t1, n1 = run(nq)
+    t2, n2 = run(nq)
+    t = min(t1, t2)
+    return {"value": n1 / t, "unit": "queries/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{n1} queries (every {max(1, Qtot // nq)}-th) of the {name} workload against all "
+                      f"{b.p.shape[1]} points, plain-PyTorch autograd fwd+1st+2nd-order Eikonal step, "
+                      f"float32, best of 2 runs ({t:.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="clustered", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by "
+              "query across GPUs, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cuda_knn_aggregate import _lib, knn_aggregate_aniso
+
+    dev = torch.device("cuda:0")
+    b = make_batch(args.workload).to(dev)
+    B, Q, P, F, K = (b.shape[k] for k in ("B", "Q", "P", "F", "K"))
+    v = b.v
+    params = [b.q, b.p, b.f, b.sigma, b.R]
+    for t in params:
+        t.requires_grad_(True)
+
+    def step():
+        f_out, w_out, dists, idxs = knn_aggregate_aniso(*params, K)
+        dq = torch.autograd.grad(f_out, params[0], v.expand_as(f_out), create_graph=True)[0]
+        L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+        return torch.autograd.grad(L, params)
+
+    print(f"[bench] workload {WORKLOADS[args.workload]}: warmup {args.warmup}", flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"[bench] timing {args.steps} steps", flush=True)
+    with _lib.trace() as tr:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = B * Q / ((t1 - t0) / args.steps)
+    summ = tr.summary()
+    kernels = {k: {"launches": n, "avg_ms": tot / n, "ms_per_step": tot / args.steps}
+               for k, (n, tot) in summ.items()}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+
+    def roof(kname):
+        avg_s = kernels[kname]["avg_ms"] * 1e-3
+        if kname == "knn_topk":
+            flops = KEY_FLOP["aniso"] * B * Q * P
+            ach = flops / avg_s / 1e12
+            return {"kernel": kname, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
+                    "per_launch": f"{KEY_FLOP['aniso']} FLOP x {B * Q * P} (query, point) pairs"}
+        byt = algorithmic_bytes(kname, B, Q, P, F, K)
+        ach = byt / avg_s / 1e9
+        return {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "per_launch": f"{byt} algorithmic bytes"}
+
+    gather = [k for k in ("aggregate", "query_grad", "point_grad", "query_grad2", "point_grad2") if k in kernels]
+    g_bytes = sum(algorithmic_bytes(k, B, Q, P, F, K) * kernels[k]["launches"] / args.steps for k in gather)
+    g_s = sum(kernels[k]["ms_per_step"] for k in gather) * 1e-3
+    roof_agg = {"kernels": gather, "bound": "hbm", "achieved": g_bytes / g_s / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": g_bytes / g_s / 1e9 / HBM_PEAK_GBS, "ms_per_step": g_s * 1e3,
+                "bytes_per_query": g_bytes / (B * Q)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.workload], "B": B, "Q": Q, "P": P, "K": K, "F": F,
+                   "step": "fwd + 1st-order (create_graph) + 2nd-order Eikonal backward"},
+        "roofline": roof(dom),
+        "roofline_gather": roof_agg,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,469 @@
+// C ABI entry points (declared in include/aknn.h).
+//
+// Each entry validates the problem description, then enqueues the kernels on the
+// caller's stream.  Nothing here allocates device memory or synchronises.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "aknn_common.h"
+#include "aknn_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// ------------------------------------------------------------------ tracing
+// When enabled, every library launch is bracketed by two hipEvents recorded on the
+// launch stream; aknn_trace_collect waits for them and reports per-launch times.
+// Off by default (zero cost); not for use under stream capture.
+struct TraceRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::mutex g_trace_mu;
+bool g_trace_on = false;
+std::vector<TraceRec> g_trace;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pool_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+class Span {
+ public:
+  Span(const char* name, hipStream_t st) : st_(st) {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    if (!g_trace_on) return;
+    rec_.name = name;
+    rec_.a = pool_event();
+    rec_.b = pool_event();
+    if (rec_.a && rec_.b) {
+      hipEventRecord(rec_.a, st_);
+      on_ = true;
+    }
+  }
+  ~Span() {
+    if (!on_) return;
+    hipEventRecord(rec_.b, st_);
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    g_trace.push_back(rec_);
+  }
+
+ private:
+  hipStream_t st_;
+  TraceRec rec_{};
+  bool on_ = false;
+};
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  if (e == hipErrorNotSupported)
+    return fail(AKNN_E_UNSUPPORTED,
+                "%s: feature width not supported (F <= 128, or F a multiple of 4 up to 512 "
+                "in float32 / of 2 up to 256 in float64)",
+                where);
+  return fail(AKNN_E_LAUNCH, "%s: %s", where, hipGetErrorString(e));
+}
+
+int64_t elem_size(const aknn_dims* d) { return d->dtype == AKNN_F64 ? 8 : 4; }
+
+bool ragged(const aknn_dims* d) { return d->q_off != nullptr; }
+
+// Smallest per-cloud point count (K must not exceed it).
+int64_t min_points(const aknn_dims* d) {
+  if (!ragged(d)) return d->P;
+  int64_t m = INT64_MAX;
+  for (int64_t b = 0; b < d->B; ++b) m = std::min(m, d->p_off_host[b + 1] - d->p_off_host[b]);
+  return m;
+}
+
+int64_t query_rows(const aknn_dims* d) { return ragged(d) ? d->Q : d->B * d->Q; }
+int64_t point_rows(const aknn_dims* d) { return ragged(d) ? d->P : d->B * d->P; }
+
+int check_dims(const aknn_dims* d, bool need_k) {
+  if (!d) return fail(AKNN_E_INVALID, "dims is NULL");
+  if (d->dtype != AKNN_F32 && d->dtype != AKNN_F64)
+    return fail(AKNN_E_INVALID, "dtype must be AKNN_F32 or AKNN_F64 (got %d)", d->dtype);
+  if (d->key != AKNN_KEY_ANISO && d->key != AKNN_KEY_EUCLIDEAN)
+    return fail(AKNN_E_INVALID, "unknown key mode %d", d->key);
+  if (d->B < 0 || d->Q < 0 || d->P < 0 || d->F < 1)
+    return fail(AKNN_E_INVALID, "bad sizes B=%lld Q=%lld P=%lld F=%lld", (long long)d->B,
+                (long long)d->Q, (long long)d->P, (long long)d->F);
+  const bool rq = d->q_off != nullptr, rp = d->p_off != nullptr;
+  if (rq != rp || (rq && (!d->q_off_host || !d->p_off_host)))
+    return fail(AKNN_E_INVALID, "ragged batches need device and host offsets for q and p");
+  if (rq) {
+    if (d->q_off_host[0] != 0 || d->p_off_host[0] != 0 || d->q_off_host[d->B] != d->Q ||
+        d->p_off_host[d->B] != d->P)
+      return fail(AKNN_E_INVALID, "offsets must start at 0 and end at the total row counts");
+    for (int64_t b = 0; b < d->B; ++b)
+      if (d->q_off_host[b + 1] < d->q_off_host[b] || d->p_off_host[b + 1] < d->p_off_host[b])
+        return fail(AKNN_E_INVALID, "offsets must be non-decreasing");
+  }
+  if (need_k) {
+    if (d->K < 1) return fail(AKNN_E_INVALID, "K must be >= 1 (got %lld)", (long long)d->K);
+    if (d->K > 64) return fail(AKNN_E_UNSUPPORTED, "K must be <= 64 (got %lld)", (long long)d->K);
+    if (d->B > 0 && d->K > min_points(d))
+      return fail(AKNN_E_INVALID, "K (%lld) must not exceed the number of points (%lld)",
+                  (long long)d->K, (long long)min_points(d));
+  }
+  if (point_rows(d) >= (int64_t(1) << 31) || query_rows(d) * std::max<int64_t>(d->K, 1) >= (int64_t(1) << 31))
+    return fail(AKNN_E_UNSUPPORTED, "batch too large (point rows and query rows * K must be < 2^31)");
+  return AKNN_OK;
+}
+
+aknn::Clouds clouds_of(const aknn_dims* d) {
+  aknn::Clouds c;
+  c.q_off = d->q_off;
+  c.p_off = d->p_off;
+  c.B = d->B;
+  c.Q = d->Q;
+  c.P = d->P;
+  return c;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Launch shape of the top-K scan: queries per lane and waves splitting the points.
+void pick_knn_shape(const aknn_dims* d, const aknn_tune* t, int* qpt, int* split) {
+  int qq = t && t->queries_per_lane > 0 ? t->queries_per_lane : 1;
+  int ss = t && t->point_split > 0 ? t->point_split : 0;
+  if (qq != 1 && qq != 2 && qq != 4) qq = 1;
+  if (ss == 0) {
+    // enough waves to cover the 1024 SIMDs several times over
+    const int64_t waves = (query_rows(d) + 64 * qq - 1) / (64 * qq);
+    const int64_t pmin = ragged(d) ? std::max<int64_t>(1, d->P / std::max<int64_t>(1, d->B)) : d->P;
+    ss = 1;
+    while (ss < 8 && waves * ss < 4096 && pmin / (2 * ss) >= 4 * d->K) ss *= 2;
+  }
+  if (ss != 1 && ss != 2 && ss != 4 && ss != 8) ss = 1;
+  *qpt = qq;
+  *split = ss;
+}
+
+int64_t knn_tiles(const aknn_dims* d, int qpt) {
+  int64_t n = 0;
+  for (int64_t b = 0; b < d->B; ++b) {
+    const int64_t qb = d->q_off_host[b + 1] - d->q_off_host[b];
+    n += (qb + 64 * qpt - 1) / (64 * qpt);
+  }
+  return n;
+}
+
+// Ragged tile table: tile_prefix[b] = first tile of cloud b (B+1 entries), one thread.
+__global__ void tile_prefix_kernel(const int64_t* __restrict__ q_off, int64_t B, int64_t per,
+                                   int64_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t acc = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    out[b] = acc;
+    acc += (q_off[b + 1] - q_off[b] + per - 1) / per;
+  }
+  out[B] = acc;
+}
+
+size_t rec_bytes(const aknn_dims* d) { return align256(static_cast<size_t>(point_rows(d)) * 16 * elem_size(d)); }
+
+int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
+            const void* sigma, const void* R, void* dists, int64_t* idxs, void* ws,
+            size_t ws_bytes, hipStream_t st) {
+  char* base = static_cast<char*>(ws);
+  void* rec = base;
+  hipError_t e;
+  {
+    Span sp("pack_points", st);
+    e = d->dtype == AKNN_F32 ? aknn::launch_pack_points<float>(p, sigma, R, rec, point_rows(d), st)
+                             : aknn::launch_pack_points<double>(p, sigma, R, rec, point_rows(d), st);
+  }
+  if (e != hipSuccess) return hip_fail(e, "pack_points");
+  aknn::KnnLaunch L{};
+  L.dtype = d->dtype;
+  L.key = d->key;
+  L.cl = clouds_of(d);
+  L.K = d->K;
+  pick_knn_shape(d, tune, &L.qpt, &L.split);
+  L.q = q;
+  L.rec = rec;
+  L.dists = dists;
+  L.idxs = idxs;
+  L.tiles = nullptr;
+  L.ntiles = 0;
+  if (ragged(d)) {
+    int64_t* prefix = reinterpret_cast<int64_t*>(base + rec_bytes(d));
+    hipLaunchKernelGGL(tile_prefix_kernel, dim3(1), dim3(64), 0, st, d->q_off, d->B,
+                       int64_t(64) * L.qpt, prefix);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tile_prefix");
+    L.tiles = prefix;
+    L.ntiles = knn_tiles(d, L.qpt);
+    if (L.ntiles == 0) return AKNN_OK;
+  }
+  {
+    Span sp("knn_topk", st);
+    e = aknn::launch_knn(L, st);
+  }
+  if (e != hipSuccess) return hip_fail(e, "knn_topk");
+  return AKNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aknn_abi_version(void) { return AKNN_ABI_VERSION; }
+
+void aknn_trace_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  g_trace_on = on != 0;
+}
+
+int aknn_trace_collect(char* names, size_t names_bytes, float* ms, int max_records) {
+  std::vector<TraceRec> recs;
+  {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    recs.swap(g_trace);
+  }
+  int n = 0;
+  size_t used = 0;
+  for (const TraceRec& r : recs) {
+    float t = -1.f;
+    if (hipEventSynchronize(r.b) == hipSuccess) hipEventElapsedTime(&t, r.a, r.b);
+    if (n < max_records && ms && names) {
+      const size_t len = strlen(r.name) + 1;
+      if (used + len <= names_bytes) {
+        memcpy(names + used, r.name, len);
+        used += len;
+        ms[n++] = t;
+      }
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  for (const TraceRec& r : recs) {
+    g_event_pool.push_back(r.a);
+    g_event_pool.push_back(r.b);
+  }
+  return n;
+}
+
+const char* aknn_last_error(void) { return g_last_error.c_str(); }
+
+size_t aknn_forward_workspace_size(const aknn_dims* d) {
+  if (check_dims(d, false) != AKNN_OK) return 0;
+  return rec_bytes(d) + align256(static_cast<size_t>(d->B + 1) * 8);
+}
+
+int aknn_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
+             const void* sigma, const void* R, void* dists, int64_t* idxs, void* ws,
+             size_t ws_bytes, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!q || !p || !sigma || !R || !dists || !idxs) return fail(AKNN_E_INVALID, "null tensor pointer");
+  if (!ws || ws_bytes < aknn_forward_workspace_size(d))
+    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
+                aknn_forward_workspace_size(d));
+  return run_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
+                 const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
+                 void* dists, int64_t* idxs, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!f || !f_out || !w_out) return fail(AKNN_E_INVALID, "null tensor pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  rc = aknn_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, stream);
+  if (rc) return rc;
+  aknn::QueryLaunch L{};
+  L.dtype = d->dtype;
+  L.key = d->key;
+  L.cl = clouds_of(d);
+  L.rows = query_rows(d);
+  L.F = d->F;
+  L.K = d->K;
+  L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+  L.idxs = idxs;
+  L.dists = dists;
+  L.dists_out = dists;  // Euclidean ranking: overwritten with the anisotropic D
+  L.f_out = f_out;
+  L.w_out = w_out;
+  hipError_t e;
+  {
+    Span sp("aggregate", st);
+    e = aknn::launch_aggregate(L, st);
+  }
+  if (e != hipSuccess) return hip_fail(e, "aggregate");
+  return AKNN_OK;
+}
+
+int aknn_aggregate(const aknn_dims* d, const void* q, const void* p, const void* f,
+                   const void* sigma, const void* R, const int64_t* idxs, void* f_out,
+                   void* w_out, void* dists, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!q || !p || !f || !sigma || !R || !idxs || !f_out || !w_out || !dists)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  aknn::QueryLaunch L{};
+  L.dtype = d->dtype;
+  L.key = d->key;
+  L.cl = clouds_of(d);
+  L.rows = query_rows(d);
+  L.F = d->F;
+  L.K = d->K;
+  L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+  L.idxs = idxs;
+  L.dists = nullptr;
+  L.dists_out = dists;
+  L.f_out = f_out;
+  L.w_out = w_out;
+  hipError_t e;
+  {
+    Span sp("aggregate_precomputed", static_cast<hipStream_t>(stream));
+    e = aknn::launch_aggregate(L, static_cast<hipStream_t>(stream));
+  }
+  if (e != hipSuccess) return hip_fail(e, "aggregate");
+  return AKNN_OK;
+}
+
+size_t aknn_csr_workspace_size(const aknn_dims* d) {
+  if (check_dims(d, false) != AKNN_OK) return 0;
+  size_t bytes = 0;
+  const hipError_t e = aknn::csr_temp_bytes(query_rows(d) * d->K, point_rows(d), &bytes);
+  if (e != hipSuccess) {
+    hip_fail(e, "csr_workspace_size");
+    return 0;
+  }
+  return bytes;
+}
+
+int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets,
+                   int32_t* csr_entries, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (!csr_offsets || (query_rows(d) > 0 && (!idxs || !csr_entries)))
+    return fail(AKNN_E_INVALID, "null pointer");
+  const size_t need = aknn_csr_workspace_size(d);
+  if (need == 0) return AKNN_E_LAUNCH;  // message set by the size query
+  if (!ws || ws_bytes < need)
+    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes, need);
+  hipError_t e;
+  {
+    Span sp("build_csr", static_cast<hipStream_t>(stream));
+    e = aknn::launch_build_csr(idxs, clouds_of(d), query_rows(d), d->K, point_rows(d), csr_offsets,
+                               csr_entries, ws, ws_bytes, static_cast<hipStream_t>(stream));
+  }
+  if (e != hipSuccess) return hip_fail(e, "build_csr");
+  return AKNN_OK;
+}
+
+static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, const void* g,
+                           const void* q, const void* p, const void* f, const void* sigma,
+                           const void* R, const int64_t* idxs, const int32_t* csr_off,
+                           const int32_t* csr_ent, void* dgf, void* dq, void* dp, void* df,
+                           void* dsigma, void* dR, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool second = gg != nullptr;
+  if (!g || !q || !p || !f || !sigma || !R || (query_rows(d) > 0 && !idxs))
+    return fail(AKNN_E_INVALID, "null input pointer");
+  if ((needs & AKNN_NEED_DQ) && !dq) return fail(AKNN_E_INVALID, "dq requested but NULL");
+  if ((needs & AKNN_NEED_DGRAD_F_OUT) && !dgf) return fail(AKNN_E_INVALID, "d_grad_f_out requested but NULL");
+  if ((needs & AKNN_NEED_DP) && !dp) return fail(AKNN_E_INVALID, "dp requested but NULL");
+  if ((needs & AKNN_NEED_DF) && !df) return fail(AKNN_E_INVALID, "df requested but NULL");
+  if ((needs & AKNN_NEED_DSIGMA) && !dsigma) return fail(AKNN_E_INVALID, "dsigma requested but NULL");
+  if ((needs & AKNN_NEED_DR) && !dR) return fail(AKNN_E_INVALID, "dR requested but NULL");
+  const bool want_q = (needs & AKNN_NEED_DQ) || (second && (needs & AKNN_NEED_DGRAD_F_OUT));
+  if (want_q && query_rows(d) > 0) {
+    aknn::QueryLaunch L{};
+    L.dtype = d->dtype;
+    L.key = d->key;
+    L.cl = clouds_of(d);
+    L.rows = query_rows(d);
+    L.F = d->F;
+    L.K = d->K;
+    L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+    L.idxs = idxs;
+    L.grad_f_out = g;
+    L.grad_grad_q = gg;
+    L.dq = (needs & AKNN_NEED_DQ) ? dq : nullptr;
+    L.d_grad_f_out = (needs & AKNN_NEED_DGRAD_F_OUT) ? dgf : nullptr;
+    hipError_t e;
+    {
+      Span sp(second ? "query_grad2" : "query_grad", st);
+      e = aknn::launch_query_grad(L, second, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, second ? "query_grad2" : "query_grad");
+  }
+  if ((needs & AKNN_NEED_POINTS) && point_rows(d) > 0) {
+    if (!csr_off || (query_rows(d) > 0 && !csr_ent))
+      return fail(AKNN_E_INVALID, "point gradients need the CSR from aknn_build_csr");
+    aknn::PointLaunch L{};
+    L.dtype = d->dtype;
+    L.key = d->key;
+    L.rows = point_rows(d);
+    L.F = d->F;
+    L.K = d->K;
+    L.csr_off = csr_off;
+    L.csr_ent = csr_ent;
+    L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+    L.grad_f_out = g;
+    L.grad_grad_q = gg;
+    L.dp = (needs & AKNN_NEED_DP) ? dp : nullptr;
+    L.df = (needs & AKNN_NEED_DF) ? df : nullptr;
+    L.dsigma = (needs & AKNN_NEED_DSIGMA) ? dsigma : nullptr;
+    L.dR = (needs & AKNN_NEED_DR) ? dR : nullptr;
+    hipError_t e;
+    {
+      Span sp(second ? "point_grad2" : "point_grad", st);
+      e = aknn::launch_point_grad(L, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, second ? "point_grad2" : "point_grad");
+  }
+  return AKNN_OK;
+}
+
+int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* q,
+                  const void* p, const void* f, const void* sigma, const void* R,
+                  const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
+                  void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
+  return backward_common(d, needs & ~AKNN_NEED_DGRAD_F_OUT, nullptr, grad_f_out, q, p, f, sigma, R,
+                         idxs, csr_offsets, csr_entries, nullptr, dq, dp, df, dsigma, dR, stream);
+}
+
+int aknn_backward2(const aknn_dims* d, uint32_t needs, const void* grad_grad_q,
+                   const void* grad_f_out, const void* q, const void* p, const void* f,
+                   const void* sigma, const void* R, const int64_t* idxs,
+                   const int32_t* csr_offsets, const int32_t* csr_entries, void* d_grad_f_out,
+                   void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
+  if (!grad_grad_q) return fail(AKNN_E_INVALID, "grad_grad_q is NULL");
+  return backward_common(d, needs, grad_grad_q, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets,
+                         csr_entries, d_grad_f_out, dq, dp, df, dsigma, dR, stream);
+}
+
+}  // extern "C"

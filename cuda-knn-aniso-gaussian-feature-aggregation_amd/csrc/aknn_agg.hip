@@ -1,0 +1,278 @@
+// Query-side gather kernels for gfx950: forward aggregation, first-order dq and
+// second-order (dq, d grad_f_out).
+//
+// Replaces the per-query loops of
+//   knn_aggregate_aniso_forward_cuda_kernel      csrc/forward.cu:99-109
+//   knn_aggregate_aniso_backward_cuda_kernel     csrc/backward.cu:31-98  (dq part)
+//   knn_aggregate_aniso_backward_2nd_cuda_kernel csrc/backward.cu:156-305 (dq, dgrad_f_out)
+//
+// Mapping: one row of 16 lanes per query (4 queries per wave).  Lane r owns
+// neighbour slot k = kb + r of each batch of 16 slots: it loads that slot's index,
+// recomputes the slot's geometry (never read back from memory) and, after the
+// cross-lane reduce-scatter, holds that slot's a_k = grad_f_out[q] . f[n_k].
+// Features are spread over the row (VEC consecutive values per lane and chunk), so
+// every f row gather is one contiguous, coalesced read of F values.
+#include <type_traits>
+
+#include "aknn_common.h"
+#include "aknn_internal.h"
+
+namespace aknn {
+
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC) VecT {
+  T v[VEC];
+};
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vload(T (&d)[VEC], const T* __restrict__ src) {
+  const VecT<T, VEC> x = *reinterpret_cast<const VecT<T, VEC>*>(src);
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) d[i] = x.v[i];
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T* __restrict__ dst, const T (&s)[VEC]) {
+  VecT<T, VEC> x;
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) x.v[i] = s[i];
+  *reinterpret_cast<VecT<T, VEC>*>(dst) = x;
+}
+
+// Broadcast of lane J of each 16-lane row (ds_swizzle, bit mode: and 0x10, or J).
+template <int J>
+__device__ __forceinline__ int row_get(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, (J << 5) | 0x10);
+}
+template <int J>
+__device__ __forceinline__ float row_get(float v) {
+  return __int_as_float(row_get<J>(__float_as_int(v)));
+}
+template <int J>
+__device__ __forceinline__ double row_get(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = row_get<J>(static_cast<int>(b & 0xffffffffll));
+  const int hi = row_get<J>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+enum { kFwd = 0, kBwd1 = 1, kBwd2 = 2 };
+
+// Unrolled loop over the 16 slots of a batch (J must be a compile-time constant for
+// the swizzle pattern).
+template <int J, int N, typename F>
+__device__ __forceinline__ void static_for(F&& fn) {
+  if constexpr (J < N) {
+    fn(std::integral_constant<int, J>{});
+    static_for<J + 1, N>(fn);
+  }
+}
+
+template <typename T, int VEC, int CH, int KEY, int MODE>
+__global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
+  const T* __restrict__ q = static_cast<const T*>(L.q);
+  const T* __restrict__ p = static_cast<const T*>(L.p);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const T* __restrict__ sg = static_cast<const T*>(L.sigma);
+  const T* __restrict__ R = static_cast<const T*>(L.R);
+  const int64_t* __restrict__ idxs = L.idxs;
+  const int64_t F = L.F;
+  const int K = static_cast<int>(L.K);
+
+  const int r = threadIdx.x & 15;
+  const int64_t row_raw = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const bool row_ok = row_raw < L.rows;
+  const int64_t row = row_ok ? row_raw : L.rows - 1;  // keep the row's lanes active
+  const int64_t b = L.cl.cloud_of_query(row);
+  const int64_t pb = L.cl.p_begin(b);
+
+  const T qx = q[row * 3 + 0], qy = q[row * 3 + 1], qz = q[row * 3 + 2];
+
+  // feature slice of this lane: chunk c covers [(c*16 + r)*VEC, +VEC)
+  T acc[CH][VEC];
+  T gsl[CH][VEC];
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc[c][i] = T(0);
+  if constexpr (MODE != kFwd) {
+    const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+      if (ft < F) vload<T, VEC>(gsl[c], g + row * F + ft);
+      else
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) gsl[c][i] = T(0);
+    }
+  }
+  T hx = T(0), hy = T(0), hz = T(0);
+  if constexpr (MODE == kBwd2) {
+    const T* __restrict__ gg = static_cast<const T*>(L.grad_grad_q);
+    hx = gg[row * 3 + 0];
+    hy = gg[row * 3 + 1];
+    hz = gg[row * 3 + 2];
+  }
+  T dqx = T(0), dqy = T(0), dqz = T(0);
+
+  for (int kb = 0; kb < K; kb += 16) {
+    const int k = kb + r;
+    const bool have = k < K;
+    const int n = have ? static_cast<int>(idxs[row * K + k]) : 0;
+    T w = T(0), coef = T(0);
+    Geo<T> geo{};
+    Pt<T> pt{};
+    if constexpr (MODE == kFwd) {
+      if (have) {
+        T D;
+        if (L.dists && KEY == AKNN_KEY_ANISO) {  // the scan's keys are the weights' D
+          D = static_cast<const T*>(L.dists)[row * K + k];
+        } else {  // given neighbours or Euclidean ranking: anisotropic D, same bits as a scan
+          pt = load_point<T>(p, sg, R, pb + n);
+          D = pair_key<T, AKNN_KEY_ANISO>(qx, qy, qz, pt.p0, pt.p1, pt.p2, pt.r[0], pt.r[1], pt.r[2],
+                                          pt.r[3], pt.r[4], pt.r[5], pt.r[6], pt.r[7], pt.r[8], pt.s0,
+                                          pt.s1, pt.s2);
+          if (row_ok) static_cast<T*>(L.dists_out)[row * K + k] = D;
+        }
+        w = gexp(-D);
+        if (row_ok) static_cast<T*>(L.w_out)[row * K + k] = w;
+      }
+      coef = w;
+    } else {
+      pt = load_point<T>(p, sg, R, pb + n);
+      geo = geometry<T>(qx, qy, qz, pt);
+      w = have ? geo.w : T(0);
+      if constexpr (MODE == kBwd2) {
+        const T s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+        coef = T(-2) * w * s;  // d grad_f_out = sum_k -2 w s f[n]
+      }
+    }
+
+    T part[16];
+    static_for<0, 16>([&](auto J) {
+      const int nj = row_get<J.value>(n);
+      const T cj = row_get<J.value>(coef);
+      const T* __restrict__ frow = f + (pb + nj) * F;
+      T dot = T(0);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+        if (ft < F) {
+          T fv[VEC];
+          vload<T, VEC>(fv, frow + ft);
+#pragma unroll
+          for (int i = 0; i < VEC; ++i) {
+            if constexpr (MODE != kBwd1) acc[c][i] += cj * fv[i];
+            if constexpr (MODE != kFwd) dot += gsl[c][i] * fv[i];
+          }
+        }
+      }
+      part[J.value] = dot;
+    });
+
+    if constexpr (MODE != kFwd) {
+      const T a = row_transpose_sum<T>(part, r);  // a for slot k
+      if (have) {
+        if constexpr (MODE == kBwd1) {
+          const T cw = T(-2) * a * w;  // dq = -2 sum a w v
+          dqx += cw * geo.v0;
+          dqy += cw * geo.v1;
+          dqz += cw * geo.v2;
+        } else {
+          // dq += 2 a w (2 s v - M h), M = R^T diag(s) R
+          const T s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+          const T rh0 = pt.r[0] * hx + pt.r[1] * hy + pt.r[2] * hz;
+          const T rh1 = pt.r[3] * hx + pt.r[4] * hy + pt.r[5] * hz;
+          const T rh2 = pt.r[6] * hx + pt.r[7] * hy + pt.r[8] * hz;
+          const T m0 = pt.s0 * rh0, m1 = pt.s1 * rh1, m2 = pt.s2 * rh2;
+          const T mh0 = pt.r[0] * m0 + pt.r[3] * m1 + pt.r[6] * m2;
+          const T mh1 = pt.r[1] * m0 + pt.r[4] * m1 + pt.r[7] * m2;
+          const T mh2 = pt.r[2] * m0 + pt.r[5] * m1 + pt.r[8] * m2;
+          const T aw2 = T(2) * a * w;
+          dqx += aw2 * (T(2) * s * geo.v0 - mh0);
+          dqy += aw2 * (T(2) * s * geo.v1 - mh1);
+          dqz += aw2 * (T(2) * s * geo.v2 - mh2);
+        }
+      }
+    }
+  }
+
+  if constexpr (MODE != kFwd) {
+    dqx = row_sum(dqx);
+    dqy = row_sum(dqy);
+    dqz = row_sum(dqz);
+    if (row_ok && L.dq && r < 3) {
+      const T val = r == 0 ? dqx : (r == 1 ? dqy : dqz);
+      static_cast<T*>(L.dq)[row * 3 + r] = val;
+    }
+  }
+  if constexpr (MODE != kBwd1) {
+    T* __restrict__ out = static_cast<T*>(MODE == kFwd ? L.f_out : L.d_grad_f_out);
+    if (row_ok && out) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+        if (ft < F) vstore<T, VEC>(out + row * F + ft, acc[c]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+// VEC: widest vector (16 B) that divides F and keeps every row aligned.
+// CH: chunks per lane so that 16 * VEC * CH >= F.
+template <typename T>
+static int pick_vec(const QueryLaunch& L) {
+  const int maxv = 16 / static_cast<int>(sizeof(T));
+  auto aligned = [&](const void* ptr, int v) {
+    return ptr == nullptr || (reinterpret_cast<uintptr_t>(ptr) % (sizeof(T) * v)) == 0;
+  };
+  for (int v = maxv; v > 1; v >>= 1) {
+    if (L.F % v == 0 && aligned(L.f, v) && aligned(L.f_out, v) && aligned(L.grad_f_out, v) &&
+        aligned(L.d_grad_f_out, v))
+      return v;
+  }
+  return 1;
+}
+
+template <typename T, int VEC, int KEY, int MODE>
+static hipError_t launch_q_ch(const QueryLaunch& L, hipStream_t st) {
+  const int64_t per = 16 * VEC;
+  const int64_t ch = (L.F + per - 1) / per;
+  const dim3 grid(static_cast<unsigned>((L.rows + 15) / 16)), block(256);
+  if (ch <= 1) hipLaunchKernelGGL((query_kernel<T, VEC, 1, KEY, MODE>), grid, block, 0, st, L);
+  else if (ch <= 2) hipLaunchKernelGGL((query_kernel<T, VEC, 2, KEY, MODE>), grid, block, 0, st, L);
+  else if (ch <= 4) hipLaunchKernelGGL((query_kernel<T, VEC, 4, KEY, MODE>), grid, block, 0, st, L);
+  else if (ch <= 8) hipLaunchKernelGGL((query_kernel<T, VEC, 8, KEY, MODE>), grid, block, 0, st, L);
+  else return hipErrorNotSupported;  // F wider than 16 * VEC * 8
+  return hipGetLastError();
+}
+
+template <typename T, int KEY, int MODE>
+static hipError_t launch_q_vec(const QueryLaunch& L, hipStream_t st) {
+  const int v = pick_vec<T>(L);
+  if constexpr (sizeof(T) == 4) {
+    if (v == 4) return launch_q_ch<T, 4, KEY, MODE>(L, st);
+  }
+  if (v == 2) return launch_q_ch<T, 2, KEY, MODE>(L, st);
+  return launch_q_ch<T, 1, KEY, MODE>(L, st);
+}
+
+template <int MODE>
+static hipError_t launch_q(const QueryLaunch& L, hipStream_t st) {
+  if (L.rows == 0) return hipSuccess;
+  if (L.dtype == AKNN_F32)
+    return L.key == AKNN_KEY_EUCLIDEAN ? launch_q_vec<float, AKNN_KEY_EUCLIDEAN, MODE>(L, st)
+                                       : launch_q_vec<float, AKNN_KEY_ANISO, MODE>(L, st);
+  return L.key == AKNN_KEY_EUCLIDEAN ? launch_q_vec<double, AKNN_KEY_EUCLIDEAN, MODE>(L, st)
+                                     : launch_q_vec<double, AKNN_KEY_ANISO, MODE>(L, st);
+}
+
+hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st) { return launch_q<kFwd>(L, st); }
+
+hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st) {
+  return second_order ? launch_q<kBwd2>(L, st) : launch_q<kBwd1>(L, st);
+}
+
+}  // namespace aknn

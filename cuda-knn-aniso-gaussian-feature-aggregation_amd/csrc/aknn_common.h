@@ -1,0 +1,220 @@
+// Shared device helpers for the gfx950 knn_aggregate_aniso kernels.
+//
+// Geometry per (query q, point n), with d = q - p_n:
+//   u = R_n d                      (csrc/forward.cu:88-90)
+//   D = s0 u0^2 + s1 u1^2 + s2 u2^2 (csrc/forward.cu:92)
+//   w = exp(-D)                    (csrc/forward.cu:102)
+//   v = R_n^T (s_n * u)            (csrc/backward.cu:207-209)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/aknn.h"
+
+namespace aknn {
+
+constexpr int kWave = 64;
+// Lanes that cooperate on one query row (or one point row) in the gather kernels:
+// one DPP row, so the cross-lane steps stay inside a row.
+constexpr int kGroup = 16;
+
+// ------------------------------------------------------------------ row addressing
+// Dense batches: cloud b owns query rows [b*Q, (b+1)*Q) and point rows [b*P, (b+1)*P).
+// Ragged batches: offsets arrays of B+1 int64 on the device.
+struct Clouds {
+  const int64_t* q_off;
+  const int64_t* p_off;
+  int64_t B, Q, P;  // dense per-cloud sizes, or totals for ragged batches
+
+  __device__ __forceinline__ int64_t cloud_of_query(int64_t row) const {
+    if (!q_off) return row / Q;
+    int64_t lo = 0, hi = B - 1;  // last b with q_off[b] <= row
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) >> 1;
+      if (q_off[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ int64_t q_begin(int64_t b) const { return q_off ? q_off[b] : b * Q; }
+  __device__ __forceinline__ int64_t q_end(int64_t b) const { return q_off ? q_off[b + 1] : (b + 1) * Q; }
+  __device__ __forceinline__ int64_t p_begin(int64_t b) const { return p_off ? p_off[b] : b * P; }
+  __device__ __forceinline__ int64_t p_end(int64_t b) const { return p_off ? p_off[b + 1] : (b + 1) * P; }
+};
+
+// ------------------------------------------------------------------ key
+// The neighbour key, one rounding per operation in the reference's association
+// order (csrc/forward.cu:85-92 and :154).  Contraction into FMA is disabled here so
+// the result is the same bits as the CPU formulation; everything else in the
+// library may contract.
+template <typename T, int KEY>
+__device__ __forceinline__ T pair_key(T qx, T qy, T qz, T px, T py, T pz, T r00, T r01, T r02,
+                                      T r10, T r11, T r12, T r20, T r21, T r22, T s0, T s1,
+                                      T s2) {
+#pragma clang fp contract(off)
+  T x = qx - px;
+  T y = qy - py;
+  T z = qz - pz;
+  if (KEY == AKNN_KEY_EUCLIDEAN) {
+    T e = x * x;
+    e = e + y * y;
+    e = e + z * z;
+    return e;
+  }
+  T j = r00 * x;
+  j = j + r01 * y;
+  j = j + r02 * z;
+  T k = r10 * x;
+  k = k + r11 * y;
+  k = k + r12 * z;
+  T l = r20 * x;
+  l = l + r21 * y;
+  l = l + r22 * z;
+  T D = (j * j) * s0;
+  D = D + (k * k) * s1;
+  D = D + (l * l) * s2;
+  return D;
+}
+
+__device__ __forceinline__ float gexp(float x) { return expf(x); }
+__device__ __forceinline__ double gexp(double x) { return exp(x); }
+
+// Per-(query, point) quantities recomputed by the backward kernels (never stored).
+template <typename T>
+struct Geo {
+  T d0, d1, d2;  // d = q - p
+  T u0, u1, u2;  // u = R d
+  T w;           // exp(-D), D by pair_key (same bits as the forward)
+  T v0, v1, v2;  // R^T (s * u)
+};
+
+// Point record as loaded by the backward kernels.
+template <typename T>
+struct Pt {
+  T p0, p1, p2;
+  T r[9];
+  T s0, s1, s2;
+};
+
+template <typename T>
+__device__ __forceinline__ Pt<T> load_point(const T* __restrict__ p, const T* __restrict__ sigma,
+                                            const T* __restrict__ R, int64_t row) {
+  Pt<T> o;
+  o.p0 = p[row * 3 + 0];
+  o.p1 = p[row * 3 + 1];
+  o.p2 = p[row * 3 + 2];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) o.r[i] = R[row * 9 + i];
+  o.s0 = sigma[row * 3 + 0];
+  o.s1 = sigma[row * 3 + 1];
+  o.s2 = sigma[row * 3 + 2];
+  return o;
+}
+
+// Weights are always anisotropic; the Euclidean key mode only changes the ranking
+// (as in the reference's lookup path, csrc/forward.cu:165-181).
+template <typename T>
+__device__ __forceinline__ Geo<T> geometry(T qx, T qy, T qz, const Pt<T>& P) {
+  Geo<T> g;
+  const T D = pair_key<T, AKNN_KEY_ANISO>(qx, qy, qz, P.p0, P.p1, P.p2, P.r[0], P.r[1], P.r[2], P.r[3],
+                               P.r[4], P.r[5], P.r[6], P.r[7], P.r[8], P.s0, P.s1, P.s2);
+  g.w = gexp(-D);
+  g.d0 = qx - P.p0;
+  g.d1 = qy - P.p1;
+  g.d2 = qz - P.p2;
+  g.u0 = P.r[0] * g.d0 + P.r[1] * g.d1 + P.r[2] * g.d2;
+  g.u1 = P.r[3] * g.d0 + P.r[4] * g.d1 + P.r[5] * g.d2;
+  g.u2 = P.r[6] * g.d0 + P.r[7] * g.d1 + P.r[8] * g.d2;
+  const T a0 = P.s0 * g.u0, a1 = P.s1 * g.u1, a2 = P.s2 * g.u2;
+  g.v0 = P.r[0] * a0 + P.r[3] * a1 + P.r[6] * a2;
+  g.v1 = P.r[1] * a0 + P.r[4] * a1 + P.r[7] * a2;
+  g.v2 = P.r[2] * a0 + P.r[5] * a1 + P.r[8] * a2;
+  return g;
+}
+
+// ------------------------------------------------------------------ cross-lane
+// DPP moves inside a row of 16 lanes (gfx9 encodings).
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              (static_cast<long long>(static_cast<unsigned>(lo))));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+
+// Partner of each lane in the 4 butterfly stages (bit 3, 2, 1, 0 of the lane in its
+// row): mirror pairs l <-> 15-l, half mirror l <-> 7-l, then quad xor 2 and xor 1.
+// Each pairing joins lanes that agree on the higher bits and differ in the stage bit.
+template <int STAGE, typename T>
+__device__ __forceinline__ T bfly(T v) {
+  if (STAGE == 3) return dpp<kDppMirror>(v);
+  if (STAGE == 2) return dpp<kDppHalfMirror>(v);
+  if (STAGE == 1) return dpp<kDppQuadXor2>(v);
+  return dpp<kDppQuadXor1>(v);
+}
+
+// Sum of v over the 16 lanes of the row, in every lane.  Fixed tree: deterministic.
+template <typename T>
+__device__ __forceinline__ T row_sum(T v) {
+  v = v + bfly<3>(v);
+  v = v + bfly<2>(v);
+  v = v + bfly<1>(v);
+  v = v + bfly<0>(v);
+  return v;
+}
+
+// Reduce-scatter of 16 partial sums: on entry lane r of a row holds x[0..15], one
+// partial per entry j; on exit lane r returns sum over the row's lanes of x[r].
+// 15 DPP moves; fixed order, so the result is deterministic.
+template <typename T>
+__device__ __forceinline__ T row_transpose_sum(T (&x)[16], int lane_in_row) {
+  // stage bit 3: keep entries whose bit 3 equals the lane's bit 3
+  const bool b3 = (lane_in_row >> 3) & 1;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const T mine = b3 ? x[8 + i] : x[i];
+    const T give = b3 ? x[i] : x[8 + i];
+    x[i] = mine + bfly<3>(give);
+  }
+  const bool b2 = (lane_in_row >> 2) & 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const T mine = b2 ? x[4 + i] : x[i];
+    const T give = b2 ? x[i] : x[4 + i];
+    x[i] = mine + bfly<2>(give);
+  }
+  const bool b1 = (lane_in_row >> 1) & 1;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const T mine = b1 ? x[2 + i] : x[i];
+    const T give = b1 ? x[i] : x[2 + i];
+    x[i] = mine + bfly<1>(give);
+  }
+  const bool b0 = lane_in_row & 1;
+  const T mine = b0 ? x[1] : x[0];
+  const T give = b0 ? x[0] : x[1];
+  return mine + bfly<0>(give);
+}
+
+// Value of v held by lane `src` (0..15) of this lane's row.
+template <typename T>
+__device__ __forceinline__ T row_bcast(T v, int src) {
+  const int lane = __lane_id();
+  return __shfl(v, (lane & ~15) | src, 64);
+}
+
+}  // namespace aknn

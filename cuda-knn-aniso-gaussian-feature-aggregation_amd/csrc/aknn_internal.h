@@ -1,0 +1,68 @@
+// Host-side launch descriptors shared by the kernel files and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aknn_common.h"
+
+namespace aknn {
+
+struct KnnLaunch {
+  int32_t dtype, key;
+  Clouds cl;
+  const int64_t* tiles;  // ragged tile table (device), or nullptr for dense grids
+  int64_t ntiles;
+  int64_t K;
+  int qpt, split;
+  const void* q;
+  const void* rec;  // packed point records [rows][16]
+  void* dists;
+  int64_t* idxs;
+};
+
+// Gather-side (query rows) work: forward aggregation, first- and second-order dq.
+struct QueryLaunch {
+  int32_t dtype, key;
+  Clouds cl;
+  int64_t rows;  // query rows in the batch
+  int64_t F, K;
+  const void *q, *p, *f, *sigma, *R;
+  const int64_t* idxs;
+  const void* dists;       // forward: keys from the top-K (nullptr: recompute from idxs)
+  void* dists_out;         // forward with recomputed keys: where to store them
+  const void* grad_f_out;  // backward
+  const void* grad_grad_q; // second order
+  void *f_out, *w_out;     // forward
+  void* dq;                // backward (first or second order)
+  void* d_grad_f_out;      // second order
+};
+
+// Point-side work over the CSR transpose of idxs.
+struct PointLaunch {
+  int32_t dtype, key;
+  int64_t rows;  // point rows in the batch
+  int64_t F, K;
+  const int32_t* csr_off;
+  const int32_t* csr_ent;
+  const void *q, *p, *f, *sigma, *R;
+  const void* grad_f_out;
+  const void* grad_grad_q;  // nullptr for first order
+  void *dp, *df, *dsigma, *dR;
+};
+
+template <typename T>
+hipError_t launch_pack_points(const void* p, const void* sigma, const void* R, void* rec,
+                              int64_t rows, hipStream_t st);
+hipError_t launch_knn(const KnnLaunch& L, hipStream_t st);
+hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st);
+hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st);
+hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
+
+// CSR build
+hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);
+hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
+                            int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
+                            size_t ws_bytes, hipStream_t st);
+
+}  // namespace aknn

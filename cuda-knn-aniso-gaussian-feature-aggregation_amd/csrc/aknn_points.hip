@@ -1,0 +1,331 @@
+// Point-side gradients for gfx950 without float atomics.
+//
+// The reference scatters every (query, slot) contribution into the point arrays with
+// float atomicAdd (csrc/backward.cu:91,102-119 first order; :291,308-326 second
+// order), so its sums depend on thread timing.  Here the neighbour table is
+// transposed once into a CSR over point rows (entries q*K+k in ascending order, built
+// by a stable radix sort), and one 16-lane row per point walks its entries in that
+// fixed order: every sum has one fixed association, so results are bitwise
+// reproducible.
+//
+// Per entry (query q, slot k, point n), recomputed from idxs (never stored):
+//   a = grad_f_out[q] . f[n],   geometry u, d, w, v as in the forward
+// First order  (csrc/backward.cu:64-119):
+//   dp += 2 a w v;  dsigma_i += -a w u_i^2;  dR_ij += -2 a w s_i u_i d_j;  df += w g
+// Second order with h = grad_grad_q[q], s = h.v, t = 2 s v - M h (csrc/backward.cu:196-326):
+//   dp += -2 a w t;  dsigma_i += 2 a w u_i (u_i s - h.R_i);
+//   dR_ij += -2 a w s_i (h_j u_i + (h.R_i) d_j - 2 s u_i d_j);  df += -2 w s g
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "aknn_common.h"
+#include "aknn_internal.h"
+
+namespace aknn {
+
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC) PVec {
+  T v[VEC];
+};
+
+template <int J>
+__device__ __forceinline__ int prow_get(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, (J << 5) | 0x10);
+}
+template <int J>
+__device__ __forceinline__ float prow_get(float v) {
+  return __int_as_float(prow_get<J>(__float_as_int(v)));
+}
+template <int J>
+__device__ __forceinline__ double prow_get(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = prow_get<J>(static_cast<int>(b & 0xffffffffll));
+  const int hi = prow_get<J>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+template <int J, int N, typename Fn>
+__device__ __forceinline__ void pstatic_for(Fn&& fn) {
+  if constexpr (J < N) {
+    fn(std::integral_constant<int, J>{});
+    pstatic_for<J + 1, N>(fn);
+  }
+}
+
+template <typename T, int VEC, int CH, int KEY, bool SECOND>
+__global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
+  const T* __restrict__ q = static_cast<const T*>(L.q);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
+  const T* __restrict__ hh = static_cast<const T*>(L.grad_grad_q);
+  const int32_t* __restrict__ off = L.csr_off;
+  const int32_t* __restrict__ ent = L.csr_ent;
+  const int64_t F = L.F;
+  const int K = static_cast<int>(L.K);
+
+  const int r = threadIdx.x & 15;
+  const int64_t row_raw = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const bool row_ok = row_raw < L.rows;
+  const int64_t row = row_ok ? row_raw : L.rows - 1;
+
+  const Pt<T> pt = load_point<T>(static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
+                                 static_cast<const T*>(L.R), row);
+  const int e0 = off[row];
+  const int e1 = row_ok ? off[row + 1] : e0;  // same trip count for the whole row
+
+  T fsl[CH][VEC], dfa[CH][VEC];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      dfa[c][i] = T(0);
+      fsl[c][i] = T(0);
+    }
+    if (ft < F) {
+      const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(f + row * F + ft);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) fsl[c][i] = x.v[i];
+    }
+  }
+  T acc[15];  // dp[3], dsigma[3], dR[9]
+#pragma unroll
+  for (int i = 0; i < 15; ++i) acc[i] = T(0);
+
+  for (int eb = e0; eb < e1; eb += 16) {
+    const int e = eb + r;
+    const bool have = e < e1;
+    const int entry = have ? ent[e] : 0;
+    const int64_t qrow = have ? static_cast<int64_t>(entry / K) : 0;
+    const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
+    const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
+    const T w = have ? geo.w : T(0);
+    T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
+    if constexpr (SECOND) {
+      hx = hh[qrow * 3 + 0];
+      hy = hh[qrow * 3 + 1];
+      hz = hh[qrow * 3 + 2];
+      s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+      coef = T(-2) * w * s;  // df += -2 w s g
+    } else {
+      coef = w;  // df += w g
+    }
+    const int qi32 = static_cast<int>(qrow);
+
+    T part[16];
+    pstatic_for<0, 16>([&](auto J) {
+      const int qj = prow_get<J.value>(qi32);
+      const T cj = prow_get<J.value>(coef);
+      const T* __restrict__ grow = g + static_cast<int64_t>(qj) * F;
+      T dot = T(0);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+        if (ft < F) {
+          const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(grow + ft);
+#pragma unroll
+          for (int i = 0; i < VEC; ++i) {
+            dfa[c][i] += cj * x.v[i];
+            dot += fsl[c][i] * x.v[i];
+          }
+        }
+      }
+      part[J.value] = dot;
+    });
+    const T a = row_transpose_sum<T>(part, r);
+
+    if (have) {
+      const T u[3] = {geo.u0, geo.u1, geo.u2};
+      const T d[3] = {geo.d0, geo.d1, geo.d2};
+      const T sg[3] = {pt.s0, pt.s1, pt.s2};
+      const T aw = a * w;
+      if constexpr (!SECOND) {
+        acc[0] += T(2) * aw * geo.v0;
+        acc[1] += T(2) * aw * geo.v1;
+        acc[2] += T(2) * aw * geo.v2;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[3 + i] += -aw * u[i] * u[i];
+          const T cr = T(-2) * aw * sg[i] * u[i];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) acc[6 + 3 * i + j] += cr * d[j];
+        }
+      } else {
+        const T h[3] = {hx, hy, hz};
+        T hr[3];  // h . R_i
+#pragma unroll
+        for (int i = 0; i < 3; ++i) hr[i] = pt.r[3 * i] * hx + pt.r[3 * i + 1] * hy + pt.r[3 * i + 2] * hz;
+        const T m0 = pt.s0 * hr[0], m1 = pt.s1 * hr[1], m2 = pt.s2 * hr[2];
+        const T mh0 = pt.r[0] * m0 + pt.r[3] * m1 + pt.r[6] * m2;
+        const T mh1 = pt.r[1] * m0 + pt.r[4] * m1 + pt.r[7] * m2;
+        const T mh2 = pt.r[2] * m0 + pt.r[5] * m1 + pt.r[8] * m2;
+        const T aw2 = T(-2) * aw;
+        acc[0] += aw2 * (T(2) * s * geo.v0 - mh0);
+        acc[1] += aw2 * (T(2) * s * geo.v1 - mh1);
+        acc[2] += aw2 * (T(2) * s * geo.v2 - mh2);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[3 + i] += T(2) * aw * u[i] * (u[i] * s - hr[i]);
+          const T cr = aw2 * sg[i];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            acc[6 + 3 * i + j] += cr * (h[j] * u[i] + hr[i] * d[j] - T(2) * s * u[i] * d[j]);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < 15; ++i) acc[i] = row_sum(acc[i]);
+  if (!row_ok) return;
+  T* dp = static_cast<T*>(L.dp);
+  T* ds = static_cast<T*>(L.dsigma);
+  T* dR = static_cast<T*>(L.dR);
+  T* df = static_cast<T*>(L.df);
+  // lane r < 15 stores accumulator r
+  T mine = acc[0];
+#pragma unroll
+  for (int i = 1; i < 15; ++i) mine = (r == i) ? acc[i] : mine;
+  if (r < 3) {
+    if (dp) dp[row * 3 + r] = mine;
+  } else if (r < 6) {
+    if (ds) ds[row * 3 + (r - 3)] = mine;
+  } else if (r < 15) {
+    if (dR) dR[row * 9 + (r - 6)] = mine;
+  }
+  if (df) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+      if (ft < F) {
+        PVec<T, VEC> x;
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) x.v[i] = dfa[c][i];
+        *reinterpret_cast<PVec<T, VEC>*>(df + row * F + ft) = x;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+template <typename T>
+static int pick_vec_points(const PointLaunch& L) {
+  const int maxv = 16 / static_cast<int>(sizeof(T));
+  auto aligned = [&](const void* ptr, int v) {
+    return ptr == nullptr || (reinterpret_cast<uintptr_t>(ptr) % (sizeof(T) * v)) == 0;
+  };
+  for (int v = maxv; v > 1; v >>= 1)
+    if (L.F % v == 0 && aligned(L.f, v) && aligned(L.df, v) && aligned(L.grad_f_out, v)) return v;
+  return 1;
+}
+
+template <typename T, int VEC, int KEY, bool SECOND>
+static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
+  const int64_t per = 16 * VEC;
+  const int64_t ch = (L.F + per - 1) / per;
+  const dim3 grid(static_cast<unsigned>((L.rows + 15) / 16)), block(256);
+  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND>), grid, block, 0, st, L);
+  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND>), grid, block, 0, st, L);
+  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND>), grid, block, 0, st, L);
+  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND>), grid, block, 0, st, L);
+  else return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+template <typename T, int KEY, bool SECOND>
+static hipError_t launch_p_vec(const PointLaunch& L, hipStream_t st) {
+  const int v = pick_vec_points<T>(L);
+  if constexpr (sizeof(T) == 4) {
+    if (v == 4) return launch_p_ch<T, 4, KEY, SECOND>(L, st);
+  }
+  if (v == 2) return launch_p_ch<T, 2, KEY, SECOND>(L, st);
+  return launch_p_ch<T, 1, KEY, SECOND>(L, st);
+}
+
+template <bool SECOND>
+static hipError_t launch_p(const PointLaunch& L, hipStream_t st) {
+  if (L.dtype == AKNN_F32)
+    return L.key == AKNN_KEY_EUCLIDEAN ? launch_p_vec<float, AKNN_KEY_EUCLIDEAN, SECOND>(L, st)
+                                       : launch_p_vec<float, AKNN_KEY_ANISO, SECOND>(L, st);
+  return L.key == AKNN_KEY_EUCLIDEAN ? launch_p_vec<double, AKNN_KEY_EUCLIDEAN, SECOND>(L, st)
+                                     : launch_p_vec<double, AKNN_KEY_ANISO, SECOND>(L, st);
+}
+
+hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
+  if (L.rows == 0) return hipSuccess;
+  return L.grad_grad_q ? launch_p<true>(L, st) : launch_p<false>(L, st);
+}
+
+// ------------------------------------------------------------------ CSR transpose
+// keys[e] = global point row of entry e = q*K + k.
+__global__ void csr_keys_kernel(const int64_t* __restrict__ idxs, Clouds cl, int64_t K,
+                                int64_t entries, uint32_t* __restrict__ keys) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= entries) return;
+  const int64_t qrow = e / K;
+  const int64_t b = cl.cloud_of_query(qrow);
+  keys[e] = static_cast<uint32_t>(cl.p_begin(b) + idxs[e]);
+}
+
+// off[i] = first position of point row i in the sorted keys (lower bound), i in [0, rows].
+__global__ void csr_offsets_kernel(const uint32_t* __restrict__ sorted, int64_t entries,
+                                   int64_t rows, int32_t* __restrict__ off) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > rows) return;
+  int64_t lo = 0, hi = entries;
+  const uint32_t key = static_cast<uint32_t>(i);
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  off[i] = static_cast<int32_t>(lo);
+}
+
+static int key_bits(int64_t rows) {
+  int bits = 1;
+  while ((int64_t(1) << bits) < rows) ++bits;
+  return bits;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes) {
+  size_t sort_bytes = 0;
+  rocprim::counting_iterator<uint32_t> vin(0);
+  const hipError_t e = rocprim::radix_sort_pairs(
+      nullptr, sort_bytes, static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+      vin, static_cast<uint32_t*>(nullptr), static_cast<size_t>(entries), 0, key_bits(point_rows),
+      nullptr);
+  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(sort_bytes);
+  return e;
+}
+
+hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
+                            int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
+                            size_t ws_bytes, hipStream_t st) {
+  const int64_t entries = query_rows * K;
+  char* base = static_cast<char*>(ws);
+  uint32_t* keys_in = reinterpret_cast<uint32_t*>(base);
+  uint32_t* keys_out = reinterpret_cast<uint32_t*>(base + align256(entries * 4));
+  void* tmp = base + 2 * align256(entries * 4);
+  size_t tmp_bytes = ws_bytes - 2 * align256(entries * 4);
+  if (entries > 0) {
+    const int bs = 256;
+    hipLaunchKernelGGL(csr_keys_kernel, dim3(static_cast<unsigned>((entries + bs - 1) / bs)), dim3(bs),
+                       0, st, idxs, cl, K, entries, keys_in);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    rocprim::counting_iterator<uint32_t> vin(0);
+    err = rocprim::radix_sort_pairs(tmp, tmp_bytes, static_cast<const uint32_t*>(keys_in), keys_out,
+                                    vin, reinterpret_cast<uint32_t*>(csr_ent),
+                                    static_cast<size_t>(entries), 0, key_bits(point_rows), st);
+    if (err != hipSuccess) return err;
+  }
+  const int bs = 256;
+  hipLaunchKernelGGL(csr_offsets_kernel, dim3(static_cast<unsigned>((point_rows + 1 + bs - 1) / bs)),
+                     dim3(bs), 0, st, keys_out, entries, point_rows, csr_off);
+  return hipGetLastError();
+}
+
+}  // namespace aknn

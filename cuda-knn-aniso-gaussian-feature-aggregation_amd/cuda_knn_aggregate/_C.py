@@ -1,0 +1,347 @@
+"""Tensor-level entry points, one per export of the reference's pybind11 module ``_C``.
+
+Reference: csrc/knn_aggregate.cpp:178-185 exports ``knn_aggregate_aniso_forward``,
+``knn_aggregate_aniso_backward`` and ``knn_aggregate_aniso_backward_2nd`` (plus the
+isotropic, lookup and precomputed variants, which are not on this path).  The
+functions below keep those names, argument orders and return lists, check their
+inputs like ``CHECK_INPUT`` (csrc/knn_aggregate.h:6-8: device and contiguity, raising
+``RuntimeError``) plus dtype, shape and ``K`` range, allocate the outputs with torch
+and run the HIP library on the current torch stream.  CPU tensors are rejected:
+there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+
+__all__ = [
+    "knn_aggregate_aniso_forward",
+    "knn_precomputed_aggregate_aniso_forward",
+    "knn_aggregate_aniso_backward",
+    "knn_aggregate_aniso_backward_2nd",
+    "knn_aniso",
+    "build_csr",
+    "make_dims",
+]
+
+_DTYPES = {torch.float32: _lib.AKNN_F32, torch.float64: _lib.AKNN_F64}
+
+
+# ----------------------------------------------------------------------------- checks
+def _check_input(t: torch.Tensor, name: str) -> None:
+    """``CHECK_INPUT`` (csrc/knn_aggregate.h:6-8): a contiguous GPU tensor."""
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} must be a tensor")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def _check_float(ts: Sequence[torch.Tensor], names: Sequence[str]) -> torch.dtype:
+    dt = ts[0].dtype
+    if dt not in _DTYPES:
+        raise RuntimeError(f"{names[0]} must be float32 or float64 (got {dt})")
+    for t, n in zip(ts, names):
+        if t.dtype != dt:
+            raise RuntimeError(f"{n} must have dtype {dt} (got {t.dtype})")
+    dev = ts[0].device
+    for t, n in zip(ts, names):
+        if t.device != dev:
+            raise RuntimeError(f"{n} must be on {dev} (got {t.device})")
+    return dt
+
+
+def _check_shapes(q, p, f, sigma, R):
+    if q.dim() != 3 or q.shape[2] != 3:
+        raise RuntimeError(f"q must have shape [B, Q, 3] (got {tuple(q.shape)})")
+    B = q.shape[0]
+    if p.dim() != 3 or p.shape[0] != B or p.shape[2] != 3:
+        raise RuntimeError(f"p must have shape [B, P, 3] (got {tuple(p.shape)})")
+    P = p.shape[1]
+    if f.dim() != 3 or f.shape[0] != B or f.shape[1] != P:
+        raise RuntimeError(f"f must have shape [B, P, F] (got {tuple(f.shape)})")
+    if sigma.dim() != 3 or tuple(sigma.shape) != (B, P, 3):
+        raise RuntimeError(f"sigma must have shape [B, P, 3] (got {tuple(sigma.shape)})")
+    if R.dim() != 4 or tuple(R.shape) != (B, P, 3, 3):
+        raise RuntimeError(f"R must have shape [B, P, 3, 3] (got {tuple(R.shape)})")
+    if f.shape[2] < 1:
+        raise RuntimeError("f must have at least one feature")
+    return B, q.shape[1], P, f.shape[2]
+
+
+def _check_k(K, P) -> int:
+    if isinstance(K, bool) or not isinstance(K, int):
+        try:
+            K = int(K)
+        except Exception as exc:  # noqa: BLE001
+            raise RuntimeError(f"K must be an int (got {K!r})") from exc
+    if K < 1:
+        raise RuntimeError(f"K must be >= 1 (got {K})")
+    if K > P:
+        raise RuntimeError(f"K ({K}) must not exceed the number of points P ({P})")
+    if K > 64:
+        raise RuntimeError(f"K must be <= 64 (got {K})")
+    return K
+
+
+def make_dims(B, Q, P, F, K, dtype, key="aniso", offsets=None) -> _lib.Dims:
+    """Fill an ``aknn_dims``.  ``offsets`` = (q_off_dev, p_off_dev, q_off_host, p_off_host)
+    for ragged batches (int64 tensors of B+1 entries; the host copies must stay alive)."""
+    d = _lib.Dims()
+    d.B, d.Q, d.P, d.F, d.K = int(B), int(Q), int(P), int(F), int(K)
+    d.dtype = _DTYPES[dtype]
+    if key not in _lib.KEYS:
+        raise RuntimeError(f"key must be one of {sorted(_lib.KEYS)} (got {key!r})")
+    d.key = _lib.KEYS[key]
+    if offsets is not None:
+        qd, pd, qh, ph = offsets
+        d.q_off, d.p_off = qd.data_ptr(), pd.data_ptr()
+        d.q_off_host, d.p_off_host = qh.data_ptr(), ph.data_ptr()
+    return d
+
+
+def _stream(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _tune(tune) -> Optional[_lib.Tune]:
+    if tune is None:
+        return None
+    t = _lib.Tune()
+    t.queries_per_lane = int(tune.get("queries_per_lane", 0))
+    t.point_split = int(tune.get("point_split", 0))
+    return t
+
+
+def _tp(t: Optional[_lib.Tune]):
+    return ctypes.byref(t) if t is not None else None
+
+
+# ----------------------------------------------------------------------------- forward
+def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tune=None):
+    """Neighbour search + aggregation -> ``[f_out, w_out, dists, idxs]``.
+
+    Reference: ``knn_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:26-47,
+    csrc/forward.cu:286-338).  ``f_out`` [B,Q,F], ``w_out``/``dists`` [B,Q,K],
+    ``idxs`` [B,Q,K] int64; neighbours ordered by (key, point index).
+    """
+    for t, n in zip((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R")):
+        _check_input(t, n)
+    dt = _check_float((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R"))
+    B, Q, P, F = _check_shapes(q, p, f, sigma, R)
+    K = _check_k(K, P)
+    dev = q.device
+    f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
+    w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
+    dists = torch.empty(B, Q, K, dtype=dt, device=dev)
+    idxs = torch.empty(B, Q, K, dtype=torch.int64, device=dev)
+    if B * Q == 0:
+        return [f_out, w_out, dists, idxs]
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt, key)
+    ws_bytes = lib.aknn_forward_workspace_size(ctypes.byref(dims))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    tn = _tune(tune)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_forward(ctypes.byref(dims), _tp(tn), q.data_ptr(), p.data_ptr(),
+                                    f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                                    f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
+                                    idxs.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
+    return [f_out, w_out, dists, idxs]
+
+
+def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, key: str = "aniso"):
+    """Aggregation given neighbour indices -> ``[f_out, w_out, dists]``.
+
+    Reference: ``knn_precomputed_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:76-99,
+    csrc/forward.cu:186-230,398-450).  ``knn_idxs`` [B,Q,K] int64 must index [0, P);
+    it is range-checked here (one device reduction) because the kernel trusts it.
+    """
+    for t, n in zip((q, p, f, sigma, R, knn_idxs), ("q", "p", "f", "sigma", "R", "knn_idxs")):
+        _check_input(t, n)
+    dt = _check_float((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R"))
+    B, Q, P, F = _check_shapes(q, p, f, sigma, R)
+    if knn_idxs.dtype != torch.int64 or knn_idxs.dim() != 3 or tuple(knn_idxs.shape[:2]) != (B, Q):
+        raise RuntimeError(f"knn_idxs must be int64 [B, Q, K] (got {tuple(knn_idxs.shape)}, {knn_idxs.dtype})")
+    if int(knn_idxs.shape[2]) != int(K):
+        raise RuntimeError(f"knn_idxs has {knn_idxs.shape[2]} neighbours but K = {K}")
+    K = _check_k(K, P)
+    dev = q.device
+    f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
+    w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
+    dists = torch.empty(B, Q, K, dtype=dt, device=dev)
+    if B * Q == 0:
+        return [f_out, w_out, dists]
+    lo, hi = torch.aminmax(knn_idxs)
+    if int(lo) < 0 or int(hi) >= P:
+        raise RuntimeError(f"knn_idxs must lie in [0, {P}) (got [{int(lo)}, {int(hi)}])")
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt, key)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_aggregate(ctypes.byref(dims), q.data_ptr(), p.data_ptr(), f.data_ptr(),
+                                      sigma.data_ptr(), R.data_ptr(), knn_idxs.data_ptr(),
+                                      f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
+                                      _stream(dev)))
+    return [f_out, w_out, dists]
+
+
+def knn_aniso(q, p, sigma, R, K, *, key: str = "aniso", tune=None):
+    """Neighbour search only -> ``[dists, idxs]`` (same order and bits as the forward)."""
+    for t, n in zip((q, p, sigma, R), ("q", "p", "sigma", "R")):
+        _check_input(t, n)
+    dt = _check_float((q, p, sigma, R), ("q", "p", "sigma", "R"))
+    f_stub = torch.empty(q.shape[0], p.shape[1] if p.dim() == 3 else 0, 1, dtype=dt, device=q.device)
+    B, Q, P, _ = _check_shapes(q, p, f_stub, sigma, R)
+    K = _check_k(K, P)
+    dev = q.device
+    dists = torch.empty(B, Q, K, dtype=dt, device=dev)
+    idxs = torch.empty(B, Q, K, dtype=torch.int64, device=dev)
+    if B * Q == 0:
+        return [dists, idxs]
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, 1, K, dt, key)
+    ws_bytes = lib.aknn_forward_workspace_size(ctypes.byref(dims))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    tn = _tune(tune)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_knn(ctypes.byref(dims), _tp(tn), q.data_ptr(), p.data_ptr(),
+                                sigma.data_ptr(), R.data_ptr(), dists.data_ptr(), idxs.data_ptr(),
+                                ws.data_ptr(), ws_bytes, _stream(dev)))
+    return [dists, idxs]
+
+
+# ----------------------------------------------------------------------------- CSR
+def build_csr(idxs: torch.Tensor, P: int, key: str = "aniso"):
+    """Point-side transpose of ``idxs`` [B,Q,K]: ``(offsets int32 [B*P+1], entries int32
+    [B*Q*K])``, each point row's entries ``q*K+k`` ascending.  Deterministic."""
+    _check_input(idxs, "idxs")
+    if idxs.dtype != torch.int64 or idxs.dim() != 3:
+        raise RuntimeError("idxs must be an int64 tensor [B, Q, K]")
+    B, Q, K = idxs.shape
+    dev = idxs.device
+    off = torch.empty(B * P + 1, dtype=torch.int32, device=dev)
+    ent = torch.empty(max(B * Q * K, 1), dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, 1, K, torch.float32, key)
+    ws_bytes = lib.aknn_csr_workspace_size(ctypes.byref(dims))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_build_csr(ctypes.byref(dims), idxs.data_ptr(), off.data_ptr(),
+                                      ent.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
+    return off, ent
+
+
+# ----------------------------------------------------------------------------- backward
+def _needs_mask(needs, names):
+    if needs is None:
+        return sum(_NEED_BITS[n] for n in names)
+    m = 0
+    for n, want in zip(names, needs):
+        if want:
+            m |= _NEED_BITS[n]
+    return m
+
+
+_NEED_BITS = {
+    "dgf": _lib.NEED_DGRAD_F_OUT,
+    "dq": _lib.NEED_DQ,
+    "dp": _lib.NEED_DP,
+    "df": _lib.NEED_DF,
+    "dsigma": _lib.NEED_DSIGMA,
+    "dR": _lib.NEED_DR,
+}
+
+
+def _bwd_common(q, p, f, sigma, R, k_idxs, grads, gnames):
+    for t, n in zip((q, p, f, sigma, R, k_idxs) + tuple(grads), ("q", "p", "f", "sigma", "R", "k_idxs") + tuple(gnames)):
+        _check_input(t, n)
+    dt = _check_float((q, p, f, sigma, R) + tuple(grads), ("q", "p", "f", "sigma", "R") + tuple(gnames))
+    B, Q, P, F = _check_shapes(q, p, f, sigma, R)
+    if k_idxs.dtype != torch.int64 or k_idxs.dim() != 3 or tuple(k_idxs.shape[:2]) != (B, Q):
+        raise RuntimeError(f"k_idxs must be int64 [B, Q, K] (got {tuple(k_idxs.shape)}, {k_idxs.dtype})")
+    K = _check_k(int(k_idxs.shape[2]), P)
+    return dt, B, Q, P, F, K
+
+
+def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, k_idxs, *,
+                                 key: str = "aniso", csr=None, needs=None):
+    """First-order gradients -> ``[dq, dp, df, dsigma, dR]``.
+
+    Reference: ``knn_aggregate_aniso_backward`` (csrc/knn_aggregate.cpp:101-132,
+    csrc/backward.cu:331-389).  Like the reference, ``grad_w_out`` does not enter
+    (csrc/backward.cu:93-94); ``w_out`` is not read either -- weights are recomputed
+    from ``k_idxs``.  ``needs`` (5 bools) skips outputs (returned as ``None``);
+    ``csr`` = :func:`build_csr` result, built here when point gradients are needed.
+    """
+    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, (grad_f_out,), ("grad_f_out",))
+    if tuple(grad_f_out.shape) != (B, Q, F):
+        raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
+    names = ("dq", "dp", "df", "dsigma", "dR")
+    mask = _needs_mask(needs, names)
+    dev = q.device
+    outs = {
+        "dq": torch.empty(B, Q, 3, dtype=dt, device=dev) if mask & _lib.NEED_DQ else None,
+        "dp": torch.empty(B, P, 3, dtype=dt, device=dev) if mask & _lib.NEED_DP else None,
+        "df": torch.empty(B, P, F, dtype=dt, device=dev) if mask & _lib.NEED_DF else None,
+        "dsigma": torch.empty(B, P, 3, dtype=dt, device=dev) if mask & _lib.NEED_DSIGMA else None,
+        "dR": torch.empty(B, P, 3, 3, dtype=dt, device=dev) if mask & _lib.NEED_DR else None,
+    }
+    if mask & _lib.NEED_POINTS and csr is None:
+        csr = build_csr(k_idxs, P, key)
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt, key)
+    off, ent = csr if csr is not None else (None, None)
+    P_ = _lib.ptr
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_backward(ctypes.byref(dims), mask, grad_f_out.data_ptr(), q.data_ptr(),
+                                     p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                                     k_idxs.data_ptr(), P_(off), P_(ent), P_(outs["dq"]),
+                                     P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
+                                     P_(outs["dR"]), _stream(dev)))
+    return [outs[n] for n in names]
+
+
+def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out,
+                                     k_idxs, *, key: str = "aniso", csr=None, needs=None):
+    """Second-order gradients of ``<grad_grad_q, dq>`` ->
+    ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
+
+    Reference: ``knn_aggregate_aniso_backward_2nd`` (csrc/knn_aggregate.cpp:135-172,
+    csrc/backward.cu:392-455).  ``needs`` (6 bools) skips outputs (``None``).
+    """
+    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, (grad_grad_q, grad_f_out),
+                                    ("grad_grad_q", "grad_f_out"))
+    if tuple(grad_f_out.shape) != (B, Q, F):
+        raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
+    if tuple(grad_grad_q.shape) != (B, Q, 3):
+        raise RuntimeError(f"grad_grad_q must have shape [B, Q, 3] (got {tuple(grad_grad_q.shape)})")
+    names = ("dgf", "dq", "dp", "df", "dsigma", "dR")
+    mask = _needs_mask(needs, names)
+    dev = q.device
+    outs = {
+        "dgf": torch.empty(B, Q, F, dtype=dt, device=dev) if mask & _lib.NEED_DGRAD_F_OUT else None,
+        "dq": torch.empty(B, Q, 3, dtype=dt, device=dev) if mask & _lib.NEED_DQ else None,
+        "dp": torch.empty(B, P, 3, dtype=dt, device=dev) if mask & _lib.NEED_DP else None,
+        "df": torch.empty(B, P, F, dtype=dt, device=dev) if mask & _lib.NEED_DF else None,
+        "dsigma": torch.empty(B, P, 3, dtype=dt, device=dev) if mask & _lib.NEED_DSIGMA else None,
+        "dR": torch.empty(B, P, 3, 3, dtype=dt, device=dev) if mask & _lib.NEED_DR else None,
+    }
+    if mask & _lib.NEED_POINTS and csr is None:
+        csr = build_csr(k_idxs, P, key)
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt, key)
+    off, ent = csr if csr is not None else (None, None)
+    P_ = _lib.ptr
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_backward2(ctypes.byref(dims), mask, grad_grad_q.data_ptr(),
+                                      grad_f_out.data_ptr(), q.data_ptr(), p.data_ptr(),
+                                      f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                                      k_idxs.data_ptr(), P_(off), P_(ent), P_(outs["dgf"]),
+                                      P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]),
+                                      P_(outs["dsigma"]), P_(outs["dR"]), _stream(dev)))
+    return [outs[n] for n in names]

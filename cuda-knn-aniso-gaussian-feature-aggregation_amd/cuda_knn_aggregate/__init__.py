@@ -1,0 +1,175 @@
+"""k-NN feature aggregation with anisotropic Gaussian weights on MI355X (gfx950).
+
+Drop-in for the ``knn_aggregate_aniso`` path of the reference package of the same name
+(cuda_knn_aggregate/__init__.py:6-32, 64-117, 216-289): same call, same
+``torch.autograd.Function`` classes, same first- and second-order backward semantics.
+Compute runs in hand-written HIP kernels (``libaknn.so``, C ABI in ``include/aknn.h``)
+on the current torch stream; CPU tensors are rejected.
+
+    f_out, w_out, dists, idxs = knn_aggregate_aniso(q, p, f, sigma, R, K)
+
+with ``w = exp(-(q-p)^T R^T diag(sigma) R (q-p))`` over the K neighbours of smallest
+anisotropic distance, ordered by (distance, point index).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _C, _lib
+
+__all__ = [
+    "knn_aggregate_aniso",
+    "KNNAnisotropicAggregate",
+    "KNNPrecomputedAnisotropicAggregate",
+    "KNNAnisotropicAggregateBackward",
+]
+
+
+def knn_aggregate_aniso(
+    q: torch.Tensor,
+    p: torch.Tensor,
+    f: torch.Tensor,
+    sigma: torch.Tensor,
+    R: torch.Tensor,
+    K: int,
+    *,
+    key: str = "aniso",
+):
+    """k-NN feature aggregate using anisotropic gaussian weights.
+
+    Args:
+        q: [B, Q, 3] query point coordinates.
+        p: [B, P, 3] surface point coordinates.
+        f: [B, P, F] feature of surface points.
+        sigma: [B, P, 3] anisotropic gaussian scales along all three axes.
+        R: [B, P, 3, 3] anisotropic gaussian rotations.
+        K: int, number of nearest neighbors (1 <= K <= min(P, 64)).
+        key: neighbour ranking, ``"aniso"`` (the reference's, csrc/forward.cu:85-92) or
+            ``"euclidean"`` (plain squared distance); weights are anisotropic either way.
+
+    Returns:
+        f_out: [B, Q, F] aggregated feature.
+        w_out: [B, Q, K] weights. Not differentiable (gradients into it are ignored).
+        dists: [B, Q, K] distances. Not differentiable.
+        idxs: [B, Q, K] k-NN indices (int64). Not differentiable.
+    """
+    return KNNAnisotropicAggregate.apply(q, p, f, sigma, R, K, key)
+
+
+def _point_needs(flags):
+    return any(flags)
+
+
+class KNNAnisotropicAggregate(torch.autograd.Function):
+    """Neighbour search + aggregation (reference: cuda_knn_aggregate/__init__.py:64-117)."""
+
+    @staticmethod
+    def forward(ctx, q, p, f, sigma, R, K, key="aniso"):
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key)
+        # Like the reference (__init__.py:89); the backward kernels recompute the
+        # weights from idxs and never read w_out.
+        ctx.save_for_backward(q, p, f, sigma, R, w_out, idxs)
+        ctx.key = key
+        ctx.set_materialize_grads(False)
+        return f_out, w_out, dists, idxs
+
+    @staticmethod
+    def backward(ctx, grad_f_out, grad_w_out, grad_dists, grad_idxs):
+        q, p, f, sigma, R, w_out, idxs = ctx.saved_tensors
+        # Only grad_f_out enters (grad_w_out and grad_dists are ignored, as in the
+        # reference, csrc/backward.cu:93-94): no grad on f_out means no gradient at all.
+        if grad_f_out is None:
+            return (None,) * 7
+        if grad_w_out is None:
+            grad_w_out = torch.zeros_like(w_out)
+        needs = tuple(ctx.needs_input_grad[:5])
+        # w_out is detached: the backward never reads it, and an edge from the
+        # second-order graph back into this node would only run it again on zeros.
+        grad_q, grad_p, grad_f, grad_sigma, grad_R = KNNAnisotropicAggregateBackward.apply(
+            grad_f_out.contiguous(),
+            grad_w_out.contiguous(),
+            q,
+            p,
+            f,
+            sigma,
+            R,
+            w_out.detach(),
+            idxs,
+            ctx.key,
+            needs,
+        )
+        return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None
+
+
+class KNNPrecomputedAnisotropicAggregate(KNNAnisotropicAggregate):
+    """Aggregation with caller-given neighbour indices [B,Q,K] (reference:
+    cuda_knn_aggregate/__init__.py:167-213, minus its debug ``torch.save``).
+    ``apply(q, p, f, sigma, R, knn_idxs, K) -> (f_out, w_out, dists)``."""
+
+    @staticmethod
+    def forward(ctx, q, p, f, sigma, R, knn_idxs, K, key="aniso"):
+        f_out, w_out, dists = _C.knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, key=key)
+        ctx.save_for_backward(q, p, f, sigma, R, w_out, knn_idxs)
+        ctx.key = key
+        ctx.set_materialize_grads(False)
+        return f_out, w_out, dists
+
+    @staticmethod
+    def backward(ctx, grad_f_out, grad_w_out, grad_dists):
+        grads = KNNAnisotropicAggregate.backward(ctx, grad_f_out, grad_w_out, grad_dists, None)
+        return grads[:5] + (None, None, None)
+
+
+class KNNAnisotropicAggregateBackward(torch.autograd.Function):
+    """First-order backward as a differentiable op, whose own backward is the
+    Eikonal second order (reference: cuda_knn_aggregate/__init__.py:216-289).
+
+    Only ``grad_grad_q`` is honoured in the second order, as in the reference
+    (__init__.py:260-289): gradients arriving on dp, df, dsigma, dR are ignored.
+    """
+
+    @staticmethod
+    def forward(ctx, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key="aniso",
+                needs=None):
+        if needs is None:
+            needs = (True,) * 5
+        P = p.shape[1]
+        csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) else None
+        dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward(
+            grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs)
+        saved = [grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs]
+        if csr is not None:
+            saved += list(csr)
+        ctx.save_for_backward(*saved)
+        ctx.has_csr = csr is not None
+        ctx.key = key
+        return dq, dp, df, dsigma, dR
+
+    @staticmethod
+    def backward(ctx, grad_grad_q, grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R):
+        saved = ctx.saved_tensors
+        grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs = saved[:9]
+        csr = (saved[9], saved[10]) if ctx.has_csr else None
+        ni = ctx.needs_input_grad
+        # outputs: d grad_f_out, dq, dp, df, dsigma, dR
+        needs = (ni[0], ni[2], ni[3], ni[4], ni[5], ni[6])
+        if grad_grad_q is None or not any(needs):
+            return (None,) * 11
+        if csr is None and _point_needs(needs[2:]):
+            csr = _C.build_csr(idxs, p.shape[1], ctx.key)
+        d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
+            grad_grad_q.contiguous(),
+            grad_f_out.contiguous(),
+            grad_w_out.contiguous(),
+            q,
+            p,
+            f,
+            sigma,
+            R,
+            w_out,
+            idxs,
+            key=ctx.key,
+            csr=csr,
+            needs=needs,
+        )
+        return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None

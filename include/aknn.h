@@ -1,0 +1,162 @@
+/*
+ * aknn.h -- C ABI of the MI355X (gfx950) library behind knn_aggregate_aniso.
+ *
+ * The library takes raw device pointers and sizes only.  It allocates nothing:
+ * every output and every workspace buffer is owned by the caller (the Python
+ * layer allocates them with torch).  All work is enqueued on the caller's HIP
+ * stream (passed as `void*`, i.e. a hipStream_t); no call synchronises the
+ * device.  Functions return 0 on success and a negative AKNN_E* code on error;
+ * aknn_last_error() then returns a message for the calling thread.
+ *
+ * Tensor layouts (row-major, contiguous), all floating tensors of one dtype:
+ *   q      [B, Q, 3]      query coordinates
+ *   p      [B, P, 3]      point coordinates
+ *   f      [B, P, F]      point features
+ *   sigma  [B, P, 3]      per-point Gaussian scales (multiply the squared offset)
+ *   R      [B, P, 3, 3]   per-point rotations, R[b][n][row][col]
+ *   f_out  [B, Q, F], w_out [B, Q, K], dists [B, Q, K], idxs [B, Q, K] (int64)
+ * Ragged batches use the same layouts with the cloud axis folded into the row
+ * axis ([sum Q_i, 3], [sum P_i, 3], ...) and device offset arrays of B+1 int64.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   aknn_forward    <- knn_aggregate_aniso_forward        csrc/knn_aggregate.cpp:26-47,
+ *                      launcher csrc/forward.cu:286-338
+ *   aknn_backward   <- knn_aggregate_aniso_backward       csrc/knn_aggregate.cpp:101-132,
+ *                      launcher csrc/backward.cu:331-389
+ *   aknn_backward2  <- knn_aggregate_aniso_backward_2nd   csrc/knn_aggregate.cpp:135-172,
+ *                      launcher csrc/backward.cu:392-455
+ *   (pybind11 table csrc/knn_aggregate.cpp:178-185)
+ */
+#ifndef AKNN_H_
+#define AKNN_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define AKNN_API __attribute__((visibility("default")))
+#else
+#define AKNN_API
+#endif
+
+#define AKNN_ABI_VERSION 1
+
+/* error codes */
+#define AKNN_OK 0
+#define AKNN_E_INVALID -1   /* bad argument (shape, K range, null pointer, ...) */
+#define AKNN_E_WORKSPACE -2 /* workspace too small */
+#define AKNN_E_LAUNCH -3    /* HIP launch / runtime error */
+#define AKNN_E_UNSUPPORTED -4
+
+/* dtype */
+#define AKNN_F32 0
+#define AKNN_F64 1
+
+/* neighbour key */
+#define AKNN_KEY_ANISO 0     /* D = s0 j^2 + s1 k^2 + s2 l^2, (j,k,l) = R (q - p)  (csrc/forward.cu:85-92) */
+#define AKNN_KEY_EUCLIDEAN 1 /* D = x^2 + y^2 + z^2                             (csrc/forward.cu:154)   */
+
+/* gradient selection bits for aknn_backward / aknn_backward2 */
+#define AKNN_NEED_DQ 1u
+#define AKNN_NEED_DP 2u
+#define AKNN_NEED_DF 4u
+#define AKNN_NEED_DSIGMA 8u
+#define AKNN_NEED_DR 16u
+#define AKNN_NEED_DGRAD_F_OUT 32u /* aknn_backward2 only */
+#define AKNN_NEED_POINTS (AKNN_NEED_DP | AKNN_NEED_DF | AKNN_NEED_DSIGMA | AKNN_NEED_DR)
+
+/* Problem description.  Dense batches: q_off = p_off = NULL and B, Q, P give the
+ * shape.  Ragged batches: q_off / p_off are DEVICE arrays of B+1 int64 row
+ * offsets (q_off[0] = 0, q_off[B] = total query rows), Q and P hold the total row
+ * counts.  The host copies of the offsets (q_off_host / p_off_host) are required
+ * for ragged launches: they size the launch grid and are validated on the host. */
+typedef struct aknn_dims {
+  int64_t B, Q, P, F, K;
+  int32_t dtype; /* AKNN_F32 | AKNN_F64 */
+  int32_t key;   /* AKNN_KEY_* */
+  const int64_t* q_off;      /* device, B+1, or NULL (dense) */
+  const int64_t* p_off;      /* device, B+1, or NULL (dense) */
+  const int64_t* q_off_host; /* host,   B+1, or NULL (dense) */
+  const int64_t* p_off_host; /* host,   B+1, or NULL (dense) */
+} aknn_dims;
+
+/* Tunables of the neighbour search (0 = library default for the shape). */
+typedef struct aknn_tune {
+  int32_t queries_per_lane; /* 1, 2 or 4 queries held per lane                  */
+  int32_t point_split;      /* 1, 2, 4 or 8 waves splitting one query tile's points */
+} aknn_tune;
+
+AKNN_API int aknn_abi_version(void);
+AKNN_API const char* aknn_last_error(void);
+
+/* Launch tracing (no counterpart in the reference, which has none).  While enabled,
+ * each kernel launch of the library is bracketed by hipEvents on its stream.
+ * aknn_trace_collect waits for the recorded events, writes up to max_records
+ * NUL-separated launch names into `names` and their durations (ms) into `ms`, returns
+ * the count and clears the record.  Not for use under stream capture. */
+AKNN_API void aknn_trace_enable(int on);
+AKNN_API int aknn_trace_collect(char* names, size_t names_bytes, float* ms, int max_records);
+
+/* Bytes of device workspace aknn_forward needs (packed point records). */
+AKNN_API size_t aknn_forward_workspace_size(const aknn_dims* dims);
+
+/* Neighbour search + aggregation: fills f_out, w_out, dists, idxs.  dists holds each
+ * neighbour's anisotropic D (w = exp(-D)); with AKNN_KEY_EUCLIDEAN the neighbours are
+ * ranked by Euclidean distance but D and w stay anisotropic, like the reference's
+ * lookup path (csrc/forward.cu:165-181).
+ * Replaces knn_aggregate_aniso_forward (csrc/knn_aggregate.cpp:26-47).  K must be in
+ * [1, min(P_i)] and at most 64.  `tune` may be NULL. */
+AKNN_API int aknn_forward(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
+                 const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
+                 void* dists, int64_t* idxs, void* workspace, size_t workspace_bytes,
+                 void* stream);
+
+/* Neighbour search only (idxs + dists).  Used by the aggregation-free paths. */
+AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
+             const void* sigma, const void* R, void* dists, int64_t* idxs, void* workspace,
+             size_t workspace_bytes, void* stream);
+
+/* Aggregation given neighbour indices (no search): recomputes dists with the same key
+ * expression as the search (same bits), then w_out and f_out.  Replaces
+ * knn_precomputed_aggregate_aniso_forward (csrc/knn_aggregate.cpp:76-99,
+ * csrc/forward.cu:398-450); also the second half of aknn_forward.  idxs entries
+ * must lie in [0, P_i) of their cloud. */
+AKNN_API int aknn_aggregate(const aknn_dims* dims, const void* q, const void* p, const void* f,
+                            const void* sigma, const void* R, const int64_t* idxs, void* f_out,
+                            void* w_out, void* dists, void* stream);
+
+/* Point-side transpose of idxs (CSR over the B*P point rows, each row's entries
+ * q*K+k in ascending order), needed by the point gradients of both backward
+ * passes.  csr_offsets: int32 [B*P + 1]; csr_entries: int32 [B*Q*K]. */
+AKNN_API size_t aknn_csr_workspace_size(const aknn_dims* dims);
+AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t* csr_offsets,
+                   int32_t* csr_entries, void* workspace, size_t workspace_bytes, void* stream);
+
+/* First-order backward.  Replaces knn_aggregate_aniso_backward
+ * (csrc/knn_aggregate.cpp:101-132).  Gaussian weights are recomputed from idxs;
+ * grad_w_out is not an input (the reference ignores it, csrc/backward.cu:93-94).
+ * Outputs selected by `needs` are fully overwritten; others may be NULL.  The
+ * point outputs need the CSR from aknn_build_csr. */
+AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* grad_f_out, const void* q,
+                  const void* p, const void* f, const void* sigma, const void* R,
+                  const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
+                  void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream);
+
+/* Second-order backward of <grad_grad_q, dq>.  Replaces knn_aggregate_aniso_backward_2nd
+ * (csrc/knn_aggregate.cpp:135-172); like the reference only grad_grad_q is an input
+ * (cuda_knn_aggregate/__init__.py:260-289). */
+AKNN_API int aknn_backward2(const aknn_dims* dims, uint32_t needs, const void* grad_grad_q,
+                   const void* grad_f_out, const void* q, const void* p, const void* f,
+                   const void* sigma, const void* R, const int64_t* idxs,
+                   const int32_t* csr_offsets, const int32_t* csr_entries, void* d_grad_f_out,
+                   void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKNN_H_ */

@@ -1,0 +1,247 @@
+"""GPU parity: the HIP library (through its C ABI, via the ``_C`` wrappers and the
+autograd Functions) against the CPU oracle.
+
+Bars (max|gpu - oracle| / max|oracle| per output):
+* ``idxs`` and ``dists`` bit-exact against the FP32 oracle (same key expression,
+  (key, index) order);
+* ``f_out``, ``w_out`` and every first- and second-order gradient within 1e-5 of the
+  FP64 oracle evaluated on the same neighbour indices;
+* the FP64 build within 1e-10 of the FP64 oracle.
+"""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from cuda_knn_aggregate import KNNAnisotropicAggregate, _C, knn_aggregate_aniso, workloads
+from oracle import knn_ref, plain
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+TOL64 = 1e-10
+
+
+def fwd(b, dev, key="aniso", tune=None):
+    d = b.to(dev)
+    return _C.knn_aggregate_aniso_forward(d.q, d.p, d.f, d.sigma, d.R, b.K, key=key, tune=tune)
+
+
+def oracle_knn(b, key="aniso"):
+    return knn_ref.knn_ref(b.q, b.p, b.sigma, b.R, b.K, key=key)
+
+
+def f64(b):
+    return [t.double() for t in (b.q, b.p, b.f, b.sigma, b.R)]
+
+
+def assert_rel(name, got, ref, tol):
+    err = plain.rel_err(got.cpu(), ref)
+    print(f"  {name}: rel err {err:.3e} (tol {tol:.0e})", flush=True)
+    assert err <= tol, (name, err)
+
+
+# ----------------------------------------------------------------------------- neighbours
+def aniso_dists_at(b, idxs):
+    """FP32 anisotropic keys of the given neighbours (the oracle's key expression)."""
+    out = torch.empty(idxs.shape, dtype=b.q.dtype)
+    for bi in range(b.q.shape[0]):
+        D = plain.aniso_key(b.q[bi], b.p[bi], b.sigma[bi], b.R[bi])
+        out[bi] = torch.gather(D, 1, idxs[bi])
+    return out
+
+
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+def test_knn_bitexact_readme(cuda, key):
+    """Neighbours by the chosen key; dists are the anisotropic D either way."""
+    b = workloads.readme()
+    _, _, dists, idxs = fwd(b, cuda, key)
+    rd, ri = oracle_knn(b, key)
+    assert torch.equal(idxs.cpu(), ri)
+    if key == "aniso":
+        assert torch.equal(dists.cpu(), rd)
+    assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
+
+
+def test_knn_bitexact_bench_small(cuda):
+    b = workloads.bench_small()
+    _, _, dists, idxs = fwd(b, cuda)
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri)
+    assert torch.equal(dists.cpu(), rd)
+
+
+def test_knn_bitexact_clustered_full_launch(cuda):
+    """The 1M-query / 64K-point launch; rows checked against the oracle on a sample."""
+    b = workloads.clustered()
+    _, _, dists, idxs = fwd(b, cuda)
+    rows = torch.cat([torch.arange(0, 1536), torch.arange(1536, 1 << 20, 1021)])
+    sub = workloads.Batch(b.q[:, rows], b.p, b.f, b.sigma, b.R, b.K)
+    rd, ri = oracle_knn(sub)
+    assert torch.equal(idxs.cpu()[:, rows], ri)
+    assert torch.equal(dists.cpu()[:, rows], rd)
+
+
+@pytest.mark.parametrize("qpt", [1, 2, 4])
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+def test_knn_launch_shapes(cuda, qpt, split):
+    b = workloads.uniform_batch(2, 700, 900, 16, 8, seed=3)
+    _, _, dists, idxs = fwd(b, cuda, tune={"queries_per_lane": qpt, "point_split": split})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
+@pytest.mark.parametrize("K", [1, 2, 5, 12, 16, 17, 33, 64])
+def test_knn_k_values(cuda, K):
+    b = workloads.uniform_batch(1, 300, 200, K, 4, seed=K)
+    _, _, dists, idxs = fwd(b, cuda)
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
+def test_knn_k_equals_p(cuda):
+    b = workloads.uniform_batch(2, 130, 16, 16, 4, seed=5)
+    _, _, dists, idxs = fwd(b, cuda, tune={"point_split": 4})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
+def test_knn_ties_duplicated_points(cuda):
+    b = workloads.uniform_batch(1, 400, 300, 16, 4, seed=7)
+    for t in (b.p, b.sigma, b.R):
+        t[0, 150:300] = t[0, 0:150]
+    for split in (1, 2, 8):
+        _, _, dists, idxs = fwd(b, cuda, tune={"point_split": split})
+        rd, ri = oracle_knn(b)
+        assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
+# ----------------------------------------------------------------------------- values + grads
+def _check_all(b, dev, tol, dtype=torch.float32, key="aniso"):
+    bd = b.to(dev, dtype)
+    f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(bd.q, bd.p, bd.f, bd.sigma, bd.R, b.K, key=key)
+    idx = idxs.cpu()
+    ins = f64(b.to("cpu", dtype))
+    rf, rw, rD = plain.forward_given_idxs(*ins, idx)
+    assert_rel("f_out", f_out, rf, tol)
+    assert_rel("w_out", w_out, rw, tol)
+    g = torch.randn(b.q.shape[0], b.q.shape[1], b.f.shape[2], generator=torch.Generator().manual_seed(1))
+    h = torch.randn(b.q.shape[0], b.q.shape[1], 3, generator=torch.Generator().manual_seed(2))
+    g, h = g.to(dtype), h.to(dtype)
+    gd, hd = g.to(dev), h.to(dev)
+    got1 = _C.knn_aggregate_aniso_backward(gd, w_out, bd.q, bd.p, bd.f, bd.sigma, bd.R, w_out, idxs, key=key)
+    ref1 = plain.backward_closed(g.double(), *ins, idx)
+    for n, x, y in zip(("dq", "dp", "df", "dsigma", "dR"), got1, ref1):
+        assert_rel(n, x, y, tol)
+    got2 = _C.knn_aggregate_aniso_backward_2nd(hd, gd, w_out, bd.q, bd.p, bd.f, bd.sigma, bd.R, w_out, idxs, key=key)
+    ref2 = plain.backward2_closed(h.double(), g.double(), *ins, idx)
+    for n, x, y in zip(("dgrad_f_out", "dq2", "dp2", "df2", "dsigma2", "dR2"), got2, ref2):
+        assert_rel(n, x, y, tol)
+    return idx
+
+
+def test_values_and_grads_readme(cuda):
+    _check_all(workloads.readme(), cuda, TOL32)
+
+
+def test_values_and_grads_sharp_sigma(cuda):
+    """sigma in [5, 200]: weights spread over (0, 1] and gradients are large."""
+    b = workloads.uniform_batch(2, 500, 400, 16, 64, seed=11, sigma_range=(5.0, 200.0))
+    _check_all(b, cuda, TOL32)
+
+
+def test_values_and_grads_bench_small(cuda):
+    _check_all(workloads.bench_small(), cuda, TOL32)
+
+
+@pytest.mark.parametrize("F", [1, 3, 16, 32, 48, 100, 128, 256])
+def test_values_and_grads_feature_widths(cuda, F):
+    b = workloads.uniform_batch(1, 200, 150, 8, F, seed=F, sigma_range=(1.0, 30.0))
+    _check_all(b, cuda, TOL32)
+
+
+def test_values_and_grads_euclidean_key(cuda):
+    _check_all(workloads.readme(seed=4), cuda, TOL32, key="euclidean")
+
+
+def test_fp64_path(cuda):
+    b = workloads.uniform_batch(2, 300, 200, 12, 16, seed=9, sigma_range=(1.0, 50.0))
+    b64 = b.to("cpu", torch.float64)
+    idx = _check_all(b, cuda, TOL64, dtype=torch.float64)
+    _, ri = knn_ref.knn_ref(b64.q, b64.p, b64.sigma, b64.R, b.K)
+    assert torch.equal(idx, ri)
+
+
+def test_deterministic_bits(cuda):
+    b = workloads.bench_small(seed=2).to(cuda)
+    outs = []
+    for _ in range(2):
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(b.q, b.p, b.f, b.sigma, b.R, b.K)
+        g1 = _C.knn_aggregate_aniso_backward(b.g, w_out, b.q, b.p, b.f, b.sigma, b.R, w_out, idxs)
+        h = b.q * 0.5
+        g2 = _C.knn_aggregate_aniso_backward_2nd(h, b.g, w_out, b.q, b.p, b.f, b.sigma, b.R, w_out, idxs)
+        outs.append([f_out, w_out, dists, idxs] + g1 + g2)
+    for x, y in zip(*outs):
+        assert torch.equal(x, y)
+
+
+# ----------------------------------------------------------------------------- autograd
+def test_autograd_eikonal_step_matches_oracle(cuda):
+    """fwd + 1st + 2nd order through the autograd Functions vs the plain FP64 autograd
+    formulation on the same neighbours."""
+    b = workloads.uniform_batch(2, 600, 300, 16, 32, seed=12, sigma_range=(2.0, 40.0))
+    ins = [t.to(cuda).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    f_out, w_out, dists, idxs = knn_aggregate_aniso(*ins, b.K)
+    v = b.v.to(cuda)
+    dq = torch.autograd.grad(f_out, ins[0], v.expand_as(f_out), create_graph=True)[0]
+    L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+    L.backward()
+
+    def sel(q, p, s, R, K, key="aniso"):
+        return None, idxs.cpu()
+
+    Lr, gr = plain.eikonal_step(*f64(b), b.K, b.v.double(), select=sel)
+    assert abs(L.item() - Lr.item()) <= TOL32 * abs(Lr.item())
+    for n, x, y in zip(("q", "p", "f", "sigma", "R"), ins, gr):
+        assert_rel("grad " + n, x.grad, y, TOL32)
+
+
+def test_autograd_first_order_matches_oracle(cuda):
+    b = workloads.readme(seed=3)
+    ins = [t.to(cuda).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    f_out, w_out, dists, idxs = knn_aggregate_aniso(*ins, b.K)
+    (f_out * b.g.to(cuda)).sum().backward()
+    ref = plain.backward_closed(b.g.double(), *f64(b), idxs.cpu())
+    for n, x, y in zip(("q", "p", "f", "sigma", "R"), ins, ref):
+        assert_rel("grad " + n, x.grad, y, TOL32)
+
+
+def test_gradcheck_fp64_dq_path(cuda):
+    """FP64 gradcheck + gradgradcheck of (q,p,f,sigma,R) -> d(f_out . v)/dq on the GPU,
+    with K = P so the neighbour set cannot change under perturbation."""
+    b = workloads.uniform_batch(1, 6, 5, 5, 3, seed=4, sigma_range=(1.0, 4.0))
+    ins = [t.to(cuda, torch.float64).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    v = torch.randn(3, dtype=torch.float64, device=cuda)
+
+    def first(q, p, f, s, R):
+        return knn_aggregate_aniso(q, p, f, s, R, 5)[0]
+
+    def dq_path(q, p, f, s, R):
+        out = knn_aggregate_aniso(q, p, f, s, R, 5)[0]
+        return torch.autograd.grad(out, q, v.expand_as(out), create_graph=True)[0]
+
+    assert torch.autograd.gradcheck(first, ins, eps=1e-6, atol=1e-8)
+    assert torch.autograd.gradcheck(dq_path, ins, eps=1e-6, atol=1e-8)
+
+
+# ----------------------------------------------------------------------------- errors
+def test_errors(cuda):
+    b = workloads.readme().to(cuda)
+    with pytest.raises(RuntimeError, match="must not exceed"):
+        knn_aggregate_aniso(b.q, b.p, b.f, b.sigma, b.R, 501)
+    with pytest.raises(RuntimeError, match="contiguous"):
+        knn_aggregate_aniso(b.q.transpose(0, 1), b.p, b.f, b.sigma, b.R, 4)
+    with pytest.raises(RuntimeError, match="dtype"):
+        knn_aggregate_aniso(b.q.double(), b.p, b.f, b.sigma, b.R, 4)
+    with pytest.raises(RuntimeError, match="shape"):
+        knn_aggregate_aniso(b.q, b.p, b.f, b.sigma[:, :10].contiguous(), b.R, 4)
