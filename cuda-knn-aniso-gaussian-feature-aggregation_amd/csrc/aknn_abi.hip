@@ -187,9 +187,62 @@ __global__ void tile_prefix_kernel(const int64_t* __restrict__ q_off, int64_t B,
 
 size_t rec_bytes(const aknn_dims* d) { return align256(static_cast<size_t>(point_rows(d)) * 16 * elem_size(d)); }
 
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
+  aknn::PrunedLaunch L{};
+  L.dtype = d->dtype;
+  L.key = d->key;
+  L.cl = clouds_of(d);
+  L.qrows = query_rows(d);
+  L.prows = point_rows(d);
+  L.K = d->K;
+  if (ragged(d)) {
+    for (int64_t b = 0; b < d->B; ++b) {
+      L.ntiles += ceil_div(d->p_off_host[b + 1] - d->p_off_host[b], 64);
+      L.nqtiles += ceil_div(d->q_off_host[b + 1] - d->q_off_host[b], 64);
+    }
+  } else {
+    L.ntiles = d->B * ceil_div(d->P, 64);
+    L.nqtiles = d->B * ceil_div(d->Q, 64);
+  }
+  return L;
+}
+
+// Default search: the tiled search once a cloud holds enough points for pruning to
+// pay (measured crossover), the brute-force scan below that.
+int pick_search(const aknn_dims* d, const aknn_tune* t) {
+  if (t && (t->search == AKNN_SEARCH_BRUTE || t->search == AKNN_SEARCH_TILES)) return t->search;
+  const int64_t pmin = min_points(d);
+  return pmin >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
+}
+
 int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
             const void* sigma, const void* R, void* dists, int64_t* idxs, void* ws,
             size_t ws_bytes, hipStream_t st) {
+  if (pick_search(d, tune) == AKNN_SEARCH_TILES) {
+    aknn::PrunedLaunch PL = pruned_launch(d);
+    PL.q = q;
+    PL.p = p;
+    PL.sigma = sigma;
+    PL.R = R;
+    PL.dists = dists;
+    PL.idxs = idxs;
+    PL.ws = ws;
+    PL.ws_bytes = ws_bytes;
+    hipError_t e;
+    {
+      Span sp("knn_tiles_prep", st);
+      e = aknn::launch_knn_pruned_prep(PL, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "knn_tiles_prep");
+    {
+      Span sp("knn_tiles", st);
+      e = aknn::launch_knn_pruned_search(PL, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "knn_tiles");
+    return AKNN_OK;
+  }
   char* base = static_cast<char*>(ws);
   void* rec = base;
   hipError_t e;
@@ -272,7 +325,14 @@ const char* aknn_last_error(void) { return g_last_error.c_str(); }
 
 size_t aknn_forward_workspace_size(const aknn_dims* d) {
   if (check_dims(d, false) != AKNN_OK) return 0;
-  return rec_bytes(d) + align256(static_cast<size_t>(d->B + 1) * 8);
+  const size_t brute = rec_bytes(d) + align256(static_cast<size_t>(d->B + 1) * 8);
+  size_t tiles = 0;
+  const hipError_t e = aknn::pruned_workspace(pruned_launch(d), &tiles);
+  if (e != hipSuccess) {
+    hip_fail(e, "forward_workspace_size");
+    return 0;
+  }
+  return std::max(brute, tiles);
 }
 
 int aknn_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,

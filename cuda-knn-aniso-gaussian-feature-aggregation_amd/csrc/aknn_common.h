@@ -36,6 +36,15 @@ struct Clouds {
     }
     return lo;
   }
+  __device__ __forceinline__ int64_t cloud_of_point(int64_t row) const {
+    if (!p_off) return row / P;
+    int64_t lo = 0, hi = B - 1;  // last b with p_off[b] <= row
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) >> 1;
+      if (p_off[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
   __device__ __forceinline__ int64_t q_begin(int64_t b) const { return q_off ? q_off[b] : b * Q; }
   __device__ __forceinline__ int64_t q_end(int64_t b) const { return q_off ? q_off[b + 1] : (b + 1) * Q; }
   __device__ __forceinline__ int64_t p_begin(int64_t b) const { return p_off ? p_off[b] : b * P; }

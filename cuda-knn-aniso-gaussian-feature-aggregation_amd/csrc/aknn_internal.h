@@ -21,6 +21,21 @@ struct KnnLaunch {
   int64_t* idxs;
 };
 
+// Tile-pruned top-K (same results as KnnLaunch's brute-force scan).
+struct PrunedLaunch {
+  int32_t dtype, key;
+  Clouds cl;
+  int64_t qrows, prows;   // query and point rows of the batch
+  int64_t ntiles;         // point tiles (sum over clouds of ceil(P_b / 64))
+  int64_t nqtiles;        // query tiles (ragged: sum of ceil(Q_b / 64))
+  int64_t K;
+  const void *q, *p, *sigma, *R;
+  void* dists;
+  int64_t* idxs;
+  void* ws;
+  size_t ws_bytes;
+};
+
 // Gather-side (query rows) work: forward aggregation, first- and second-order dq.
 struct QueryLaunch {
   int32_t dtype, key;
@@ -58,6 +73,12 @@ hipError_t launch_knn(const KnnLaunch& L, hipStream_t st);
 hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st);
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st);
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
+
+hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes);
+hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st);
+hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st);
+// spatial order of the query rows (valid after the prep, inside the workspace)
+const uint32_t* pruned_query_order(const PrunedLaunch& L);
 
 // CSR build
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);

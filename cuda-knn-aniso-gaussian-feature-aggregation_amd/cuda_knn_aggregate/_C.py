@@ -115,6 +115,7 @@ def _tune(tune) -> Optional[_lib.Tune]:
     t = _lib.Tune()
     t.queries_per_lane = int(tune.get("queries_per_lane", 0))
     t.point_split = int(tune.get("point_split", 0))
+    t.search = _lib.SEARCH[tune.get("search", "auto")]
     return t
 
 

@@ -26,6 +26,7 @@ SOURCES = [
     "aknn_knn_f64_euclid.hip",
     "aknn_agg.hip",
     "aknn_points.hip",
+    "aknn_tiles.hip",
 ]
 ARCH = "gfx950"
 

@@ -73,7 +73,11 @@ class Dims(ctypes.Structure):
 class Tune(ctypes.Structure):
     """``aknn_tune`` (include/aknn.h)."""
 
-    _fields_ = [("queries_per_lane", ctypes.c_int32), ("point_split", ctypes.c_int32)]
+    _fields_ = [("queries_per_lane", ctypes.c_int32), ("point_split", ctypes.c_int32),
+                ("search", ctypes.c_int32)]
+
+
+SEARCH = {"auto": 0, "brute": 1, "tiles": 2}
 
 
 _lock = threading.Lock()

@@ -84,10 +84,16 @@ typedef struct aknn_dims {
   const int64_t* p_off_host; /* host,   B+1, or NULL (dense) */
 } aknn_dims;
 
+/* Neighbour search algorithms (same results, different cost). */
+#define AKNN_SEARCH_AUTO 0
+#define AKNN_SEARCH_BRUTE 1  /* scan every point of the cloud                       */
+#define AKNN_SEARCH_TILES 2  /* spatially sorted tiles, provably-useless tiles skipped */
+
 /* Tunables of the neighbour search (0 = library default for the shape). */
 typedef struct aknn_tune {
-  int32_t queries_per_lane; /* 1, 2 or 4 queries held per lane                  */
-  int32_t point_split;      /* 1, 2, 4 or 8 waves splitting one query tile's points */
+  int32_t queries_per_lane; /* brute: 1, 2 or 4 queries held per lane                  */
+  int32_t point_split;      /* brute: 1, 2, 4 or 8 waves splitting one tile's points   */
+  int32_t search;           /* AKNN_SEARCH_*                                            */
 } aknn_tune;
 
 AKNN_API int aknn_abi_version(void);
@@ -101,7 +107,9 @@ AKNN_API const char* aknn_last_error(void);
 AKNN_API void aknn_trace_enable(int on);
 AKNN_API int aknn_trace_collect(char* names, size_t names_bytes, float* ms, int max_records);
 
-/* Bytes of device workspace aknn_forward needs (packed point records). */
+/* Bytes of device workspace aknn_forward / aknn_knn need (enough for either search
+ * algorithm).  Returns 0 and sets the error message if the size query fails (for
+ * example without a usable device). */
 AKNN_API size_t aknn_forward_workspace_size(const aknn_dims* dims);
 
 /* Neighbour search + aggregation: fills f_out, w_out, dists, idxs.  dists holds each

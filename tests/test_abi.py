@@ -55,6 +55,9 @@ def test_forward_rejects_small_workspace():
     lib = _lib.load()
     d = _dims()
     need = lib.aknn_forward_workspace_size(ctypes.byref(d))
+    if need == 0:  # the sort-size query needs a device; the error names the query
+        assert "workspace_size" in lib.aknn_last_error().decode()
+        pytest.skip("no device for the workspace size query")
     assert need >= 2 * 8 * 16 * 4
     fake = ctypes.c_void_p(0x1000)
     rc = lib.aknn_forward(ctypes.byref(d), None, *([fake] * 9), fake, need - 1, None)

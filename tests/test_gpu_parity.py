@@ -63,23 +63,66 @@ def test_knn_bitexact_readme(cuda, key):
     assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
 
 
-def test_knn_bitexact_bench_small(cuda):
+SEARCHES = ["brute", "tiles"]
+
+
+@pytest.mark.parametrize("search", SEARCHES)
+def test_knn_bitexact_bench_small(cuda, search):
     b = workloads.bench_small()
-    _, _, dists, idxs = fwd(b, cuda)
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": search})
     rd, ri = oracle_knn(b)
     assert torch.equal(idxs.cpu(), ri)
     assert torch.equal(dists.cpu(), rd)
 
 
-def test_knn_bitexact_clustered_full_launch(cuda):
+@pytest.mark.parametrize("search", SEARCHES)
+def test_knn_bitexact_clustered_full_launch(cuda, search):
     """The 1M-query / 64K-point launch; rows checked against the oracle on a sample."""
     b = workloads.clustered()
-    _, _, dists, idxs = fwd(b, cuda)
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": search})
     rows = torch.cat([torch.arange(0, 1536), torch.arange(1536, 1 << 20, 1021)])
     sub = workloads.Batch(b.q[:, rows], b.p, b.f, b.sigma, b.R, b.K)
     rd, ri = oracle_knn(sub)
     assert torch.equal(idxs.cpu()[:, rows], ri)
     assert torch.equal(dists.cpu()[:, rows], rd)
+
+
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+@pytest.mark.parametrize("K", [1, 7, 16, 40, 64])
+def test_knn_tiles_k_and_key(cuda, K, key):
+    b = workloads.clustered_batch(Q=3000, P=5000, K=K, F=4, seed=K)
+    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": "tiles"})
+    rd, ri = oracle_knn(b, key)
+    assert torch.equal(idxs.cpu(), ri)
+    assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
+
+
+@pytest.mark.parametrize("search", SEARCHES)
+def test_knn_readme_literal_inputs(cuda, search):
+    """The reference README's own inputs: R = torch.rand (not a rotation) and
+    sigma = torch.rand (scales near 0 included)."""
+    g = torch.Generator().manual_seed(21)
+    B, Q, P, K, F = 2, 1000, 500, 12, 32
+    b = workloads.Batch(torch.rand(B, Q, 3, generator=g), torch.rand(B, P, 3, generator=g),
+                        torch.rand(B, P, F, generator=g), torch.rand(B, P, 3, generator=g),
+                        torch.rand(B, P, 3, 3, generator=g), K)
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": search})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
+def test_knn_tiles_degenerate_scales(cuda):
+    """Zero and negative scales (tiles that must always be scanned), far outliers and
+    exact duplicates."""
+    b = workloads.clustered_batch(Q=2000, P=3000, K=16, F=4, seed=5)
+    b.sigma[0, 100:140, 1] = 0.0
+    b.sigma[0, 400:420, 2] = -0.01
+    b.p[0, 900:905] += 50.0
+    for t in (b.p, b.sigma, b.R):
+        t[0, 2000:2400] = t[0, 1000:1400]
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": "tiles"})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
 @pytest.mark.parametrize("qpt", [1, 2, 4])
@@ -110,10 +153,10 @@ def test_knn_ties_duplicated_points(cuda):
     b = workloads.uniform_batch(1, 400, 300, 16, 4, seed=7)
     for t in (b.p, b.sigma, b.R):
         t[0, 150:300] = t[0, 0:150]
-    for split in (1, 2, 8):
-        _, _, dists, idxs = fwd(b, cuda, tune={"point_split": split})
-        rd, ri = oracle_knn(b)
-        assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+    rd, ri = oracle_knn(b)
+    for tune in ({"point_split": 1}, {"point_split": 2}, {"point_split": 8}, {"search": "tiles"}):
+        _, _, dists, idxs = fwd(b, cuda, tune=tune)
+        assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd), tune
 
 
 # ----------------------------------------------------------------------------- values + grads
@@ -170,6 +213,14 @@ def test_fp64_path(cuda):
     idx = _check_all(b, cuda, TOL64, dtype=torch.float64)
     _, ri = knn_ref.knn_ref(b64.q, b64.p, b64.sigma, b64.R, b.K)
     assert torch.equal(idx, ri)
+
+
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+def test_fp64_tiles(cuda, key):
+    b = workloads.clustered_batch(Q=2000, P=3000, K=16, F=4, seed=6).to("cpu", torch.float64)
+    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": "tiles"})
+    rd, ri = oracle_knn(b, key)
+    assert torch.equal(idxs.cpu(), ri)
 
 
 def test_deterministic_bits(cuda):
