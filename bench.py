@@ -114,7 +114,7 @@ def main():
 
     import torch
 
-    from cuda_knn_aggregate import _lib, knn_aggregate_aniso
+    from cuda_knn_aggregate import _C, _lib, knn_aggregate_aniso
 
     dev = torch.device("cuda:0")
     b = make_batch(args.workload).to(dev)
@@ -149,14 +149,28 @@ def main():
                for k, (n, tot) in summ.items()}
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
 
+    # (query, point) key evaluations of one search launch: all pairs for the brute scan,
+    # counted by the tiled search's own device counters otherwise (untimed extra call)
+    pairs = {"knn_topk": B * Q * P}
+    if "knn_tiles" in kernels:
+        st = torch.zeros(5, dtype=torch.int64, device=dev)
+        _C.knn_aniso(b.q.detach(), b.p.detach(), b.sigma.detach(), b.R.detach(), K,
+                     tune={"search": "tiles", "stats": st})
+        c = st.tolist()
+        pairs["knn_tiles"] = c[3] * 64
+        search_stats = {"waves": c[4], "super_tests_per_wave": c[0] / c[4], "tile_tests_per_wave": c[1] / c[4],
+                        "tiles_scanned_per_wave": c[2] / c[4], "points_scanned_per_query": c[3] / c[4]}
+    else:
+        search_stats = None
+
     def roof(kname):
         avg_s = kernels[kname]["avg_ms"] * 1e-3
-        if kname == "knn_topk":
-            flops = KEY_FLOP["aniso"] * B * Q * P
+        if kname in pairs:
+            flops = KEY_FLOP["aniso"] * pairs[kname]
             ach = flops / avg_s / 1e12
             return {"kernel": kname, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                    "per_launch": f"{KEY_FLOP['aniso']} FLOP x {B * Q * P} (query, point) pairs"}
+                    "per_launch": f"{KEY_FLOP['aniso']} FLOP x {pairs[kname]} evaluated (query, point) pairs"}
         byt = algorithmic_bytes(kname, B, Q, P, F, K)
         ach = byt / avg_s / 1e9
         return {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -187,6 +201,7 @@ def main():
         "roofline": roof(dom),
         "roofline_gather": roof_agg,
         "kernels": kernels,
+        "search_stats": search_stats,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", flush=True)

@@ -199,12 +199,18 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   L.K = d->K;
   if (ragged(d)) {
     for (int64_t b = 0; b < d->B; ++b) {
-      L.ntiles += ceil_div(d->p_off_host[b + 1] - d->p_off_host[b], 64);
+      const int64_t pb = d->p_off_host[b + 1] - d->p_off_host[b];
+      const int64_t tiles = ceil_div(pb, 64);
+      L.ntiles += tiles;
+      L.nsuper += ceil_div(tiles, 16);
       L.nqtiles += ceil_div(d->q_off_host[b + 1] - d->q_off_host[b], 64);
+      L.max_points = std::max(L.max_points, pb);
     }
   } else {
     L.ntiles = d->B * ceil_div(d->P, 64);
+    L.nsuper = d->B * ceil_div(ceil_div(d->P, 64), 16);
     L.nqtiles = d->B * ceil_div(d->Q, 64);
+    L.max_points = d->P;
   }
   return L;
 }
@@ -217,9 +223,14 @@ int pick_search(const aknn_dims* d, const aknn_tune* t) {
   return pmin >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
 }
 
+// Neighbour search.  On return *qorder / *porder point at spatial orders of the query
+// and point rows inside the workspace (tiled search) or are NULL (brute scan).
 int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
             const void* sigma, const void* R, void* dists, int64_t* idxs, void* ws,
-            size_t ws_bytes, hipStream_t st) {
+            size_t ws_bytes, hipStream_t st, const uint32_t** qorder = nullptr,
+            const uint32_t** porder = nullptr) {
+  if (qorder) *qorder = nullptr;
+  if (porder) *porder = nullptr;
   if (pick_search(d, tune) == AKNN_SEARCH_TILES) {
     aknn::PrunedLaunch PL = pruned_launch(d);
     PL.q = q;
@@ -230,6 +241,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.idxs = idxs;
     PL.ws = ws;
     PL.ws_bytes = ws_bytes;
+    PL.stats = tune ? static_cast<unsigned long long*>(tune->stats) : nullptr;
     hipError_t e;
     {
       Span sp("knn_tiles_prep", st);
@@ -241,6 +253,8 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
       e = aknn::launch_knn_pruned_search(PL, st);
     }
     if (e != hipSuccess) return hip_fail(e, "knn_tiles");
+    if (qorder) *qorder = aknn::pruned_query_order(PL);
+    if (porder) *porder = aknn::pruned_point_order(PL);
     return AKNN_OK;
   }
   char* base = static_cast<char*>(ws);
@@ -350,14 +364,43 @@ int aknn_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const voi
 
 int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
                  const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
-                 void* dists, int64_t* idxs, void* ws, size_t ws_bytes, void* stream) {
+                 void* dists, int64_t* idxs, void* ws, size_t ws_bytes, aknn_cache* cache,
+                 void* stream) {
   int rc = check_dims(d, true);
   if (rc) return rc;
   if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
-  if (!f || !f_out || !w_out) return fail(AKNN_E_INVALID, "null tensor pointer");
+  if (!q || !p || !sigma || !R || !f || !f_out || !w_out || !dists || !idxs)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  if (!ws || ws_bytes < aknn_forward_workspace_size(d))
+    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
+                aknn_forward_workspace_size(d));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  rc = aknn_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, stream);
+  const uint32_t *qord = nullptr, *pord = nullptr;
+  rc = run_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, st, &qord, &pord);
   if (rc) return rc;
+  // cache for the backward passes (and the aggregation below)
+  const void* rec = nullptr;
+  const int32_t* order = reinterpret_cast<const int32_t*>(qord);
+  if (cache) {
+    hipError_t e = hipSuccess;
+    if (cache->query_order) {
+      e = qord ? aknn::launch_copy_u32(qord, cache->query_order, query_rows(d), st)
+               : aknn::launch_identity(cache->query_order, query_rows(d), st);
+      if (e != hipSuccess) return hip_fail(e, "cache");
+    }
+    if (cache->point_order) {
+      e = pord ? aknn::launch_copy_u32(pord, cache->point_order, point_rows(d), st)
+               : aknn::launch_identity(cache->point_order, point_rows(d), st);
+      if (e != hipSuccess) return hip_fail(e, "cache");
+    }
+    if (cache->point_records) {
+      e = d->dtype == AKNN_F32
+              ? aknn::launch_pack_points<float>(p, sigma, R, cache->point_records, point_rows(d), st)
+              : aknn::launch_pack_points<double>(p, sigma, R, cache->point_records, point_rows(d), st);
+      if (e != hipSuccess) return hip_fail(e, "cache");
+      rec = cache->point_records;
+    }
+  }
   aknn::QueryLaunch L{};
   L.dtype = d->dtype;
   L.key = d->key;
@@ -371,6 +414,8 @@ int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const
   L.dists_out = dists;  // Euclidean ranking: overwritten with the anisotropic D
   L.f_out = f_out;
   L.w_out = w_out;
+  L.order = order;
+  L.rec = rec;
   hipError_t e;
   {
     Span sp("aggregate", st);
@@ -444,8 +489,8 @@ int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets
 static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, const void* g,
                            const void* q, const void* p, const void* f, const void* sigma,
                            const void* R, const int64_t* idxs, const int32_t* csr_off,
-                           const int32_t* csr_ent, void* dgf, void* dq, void* dp, void* df,
-                           void* dsigma, void* dR, void* stream) {
+                           const int32_t* csr_ent, const aknn_cache* cache, void* dgf, void* dq,
+                           void* dp, void* df, void* dsigma, void* dR, void* stream) {
   int rc = check_dims(d, true);
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -473,6 +518,8 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.grad_grad_q = gg;
     L.dq = (needs & AKNN_NEED_DQ) ? dq : nullptr;
     L.d_grad_f_out = (needs & AKNN_NEED_DGRAD_F_OUT) ? dgf : nullptr;
+    L.order = cache ? cache->query_order : nullptr;
+    L.rec = cache ? cache->point_records : nullptr;
     hipError_t e;
     {
       Span sp(second ? "query_grad2" : "query_grad", st);
@@ -498,6 +545,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.df = (needs & AKNN_NEED_DF) ? df : nullptr;
     L.dsigma = (needs & AKNN_NEED_DSIGMA) ? dsigma : nullptr;
     L.dR = (needs & AKNN_NEED_DR) ? dR : nullptr;
+    L.order = cache ? cache->point_order : nullptr;
     hipError_t e;
     {
       Span sp(second ? "point_grad2" : "point_grad", st);
@@ -511,19 +559,22 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
 int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
-                  void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
+                  const aknn_cache* cache, void* dq, void* dp, void* df, void* dsigma, void* dR,
+                  void* stream) {
   return backward_common(d, needs & ~AKNN_NEED_DGRAD_F_OUT, nullptr, grad_f_out, q, p, f, sigma, R,
-                         idxs, csr_offsets, csr_entries, nullptr, dq, dp, df, dsigma, dR, stream);
+                         idxs, csr_offsets, csr_entries, cache, nullptr, dq, dp, df, dsigma, dR,
+                         stream);
 }
 
 int aknn_backward2(const aknn_dims* d, uint32_t needs, const void* grad_grad_q,
                    const void* grad_f_out, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs,
-                   const int32_t* csr_offsets, const int32_t* csr_entries, void* d_grad_f_out,
-                   void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
+                   const int32_t* csr_offsets, const int32_t* csr_entries,
+                   const aknn_cache* cache, void* d_grad_f_out, void* dq, void* dp, void* df,
+                   void* dsigma, void* dR, void* stream) {
   if (!grad_grad_q) return fail(AKNN_E_INVALID, "grad_grad_q is NULL");
   return backward_common(d, needs, grad_grad_q, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets,
-                         csr_entries, d_grad_f_out, dq, dp, df, dsigma, dR, stream);
+                         csr_entries, cache, d_grad_f_out, dq, dp, df, dsigma, dR, stream);
 }
 
 }  // extern "C"

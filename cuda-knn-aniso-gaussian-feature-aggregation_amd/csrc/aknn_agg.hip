@@ -81,9 +81,13 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   const int K = static_cast<int>(L.K);
 
   const int r = threadIdx.x & 15;
-  const int64_t row_raw = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
-  const bool row_ok = row_raw < L.rows;
-  const int64_t row = row_ok ? row_raw : L.rows - 1;  // keep the row's lanes active
+  const int64_t gi = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const bool row_ok = gi < L.rows;
+  const int64_t gc = row_ok ? gi : L.rows - 1;  // keep the row's lanes active
+  // spatial processing order (when given) keeps the gathered rows of neighbouring
+  // queries in the same L2; results do not depend on it
+  const int64_t row = L.order ? static_cast<int64_t>(L.order[gc]) : gc;
+  const T* __restrict__ rec = static_cast<const T*>(L.rec);
   const int64_t b = L.cl.cloud_of_query(row);
   const int64_t pb = L.cl.p_begin(b);
 
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
         if (L.dists && KEY == AKNN_KEY_ANISO) {  // the scan's keys are the weights' D
           D = static_cast<const T*>(L.dists)[row * K + k];
         } else {  // given neighbours or Euclidean ranking: anisotropic D, same bits as a scan
-          pt = load_point<T>(p, sg, R, pb + n);
+          pt = fetch_point<T>(rec, p, sg, R, pb + n);
           D = pair_key<T, AKNN_KEY_ANISO>(qx, qy, qz, pt.p0, pt.p1, pt.p2, pt.r[0], pt.r[1], pt.r[2],
                                           pt.r[3], pt.r[4], pt.r[5], pt.r[6], pt.r[7], pt.r[8], pt.s0,
                                           pt.s1, pt.s2);
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
       }
       coef = w;
     } else {
-      pt = load_point<T>(p, sg, R, pb + n);
+      pt = fetch_point<T>(rec, p, sg, R, pb + n);
       geo = geometry<T>(qx, qy, qz, pt);
       w = have ? geo.w : T(0);
       if constexpr (MODE == kBwd2) {

@@ -120,6 +120,50 @@ __device__ __forceinline__ Pt<T> load_point(const T* __restrict__ p, const T* __
   return o;
 }
 
+// Point record from the packed [rows][16] array {p0 p1 p2, R00..R22, s0 s1 s2, -}:
+// 64-B (FP32) or 128-B (FP64) aligned, so four or eight 16-B loads.
+template <typename T>
+__device__ __forceinline__ Pt<T> load_record(const T* __restrict__ rec, int64_t row) {
+  Pt<T> o;
+  const T* r = rec + row * 16;
+  T v[16];
+  if constexpr (sizeof(T) == 4) {
+    const float4* r4 = reinterpret_cast<const float4*>(r);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 x = r4[i];
+      v[4 * i] = x.x;
+      v[4 * i + 1] = x.y;
+      v[4 * i + 2] = x.z;
+      v[4 * i + 3] = x.w;
+    }
+  } else {
+    const double2* r2 = reinterpret_cast<const double2*>(r);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double2 x = r2[i];
+      v[2 * i] = x.x;
+      v[2 * i + 1] = x.y;
+    }
+  }
+  o.p0 = v[0];
+  o.p1 = v[1];
+  o.p2 = v[2];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) o.r[i] = v[3 + i];
+  o.s0 = v[12];
+  o.s1 = v[13];
+  o.s2 = v[14];
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ Pt<T> fetch_point(const T* __restrict__ rec, const T* __restrict__ p,
+                                             const T* __restrict__ sigma, const T* __restrict__ R,
+                                             int64_t row) {
+  return rec ? load_record<T>(rec, row) : load_point<T>(p, sigma, R, row);
+}
+
 // Weights are always anisotropic; the Euclidean key mode only changes the ranking
 // (as in the reference's lookup path, csrc/forward.cu:165-181).
 template <typename T>

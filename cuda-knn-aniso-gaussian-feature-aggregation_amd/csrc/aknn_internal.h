@@ -27,7 +27,10 @@ struct PrunedLaunch {
   Clouds cl;
   int64_t qrows, prows;   // query and point rows of the batch
   int64_t ntiles;         // point tiles (sum over clouds of ceil(P_b / 64))
+  int64_t nsuper;         // super-tiles (sum over clouds of ceil(tiles_b / 16))
   int64_t nqtiles;        // query tiles (ragged: sum of ceil(Q_b / 64))
+  int64_t max_points;     // largest cloud (sizes the box reduction grid)
+  unsigned long long* stats;  // optional device counters (super tests, tile tests, scans, waves)
   int64_t K;
   const void *q, *p, *sigma, *R;
   void* dists;
@@ -51,6 +54,8 @@ struct QueryLaunch {
   void *f_out, *w_out;     // forward
   void* dq;                // backward (first or second order)
   void* d_grad_f_out;      // second order
+  const int32_t* order;    // optional processing order of the query rows (spatial)
+  const void* rec;         // optional packed point records [rows][16]
 };
 
 // Point-side work over the CSR transpose of idxs.
@@ -64,6 +69,7 @@ struct PointLaunch {
   const void* grad_f_out;
   const void* grad_grad_q;  // nullptr for first order
   void *dp, *df, *dsigma, *dR;
+  const int32_t* order;     // optional processing order of the point rows (spatial)
 };
 
 template <typename T>
@@ -74,11 +80,14 @@ hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st);
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st);
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
 
+hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st);
+hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st);
 hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes);
 hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st);
 hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st);
-// spatial order of the query rows (valid after the prep, inside the workspace)
+// spatial orders of the query and point rows (valid after the prep, inside the workspace)
 const uint32_t* pruned_query_order(const PrunedLaunch& L);
+const uint32_t* pruned_point_order(const PrunedLaunch& L);
 
 // CSR build
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);

@@ -65,9 +65,12 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int K = static_cast<int>(L.K);
 
   const int r = threadIdx.x & 15;
-  const int64_t row_raw = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
-  const bool row_ok = row_raw < L.rows;
-  const int64_t row = row_ok ? row_raw : L.rows - 1;
+  const int64_t gi = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const bool row_ok = gi < L.rows;
+  const int64_t gc = row_ok ? gi : L.rows - 1;
+  // spatial processing order (when given): neighbouring points share queries, so the
+  // gathered grad_f_out rows stay in L2; results do not depend on it
+  const int64_t row = L.order ? static_cast<int64_t>(L.order[gc]) : gc;
 
   const Pt<T> pt = load_point<T>(static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
                                  static_cast<const T*>(L.R), row);

@@ -30,7 +30,23 @@
 namespace aknn {
 
 constexpr int kTile = 64;
+constexpr int kSuper = 16;  // tiles per super-tile
 constexpr int kTileSentinel = 0x7fffffff;
+
+// Workspace pieces of the tiled search (see carve()).
+struct PrunedState {
+  float* box;
+  unsigned* obox;
+  uint64_t *pkeys, *pkeys_s, *qkeys, *qkeys_s;
+  uint32_t *pperm, *qperm;
+  int64_t *ptp, *qtp, *stp;
+  void* rec;
+  float *hdr, *shdr;
+  int32_t* tcnt;
+  void* tmp;
+  size_t tmp_bytes;
+  size_t total;
+};
 
 // Neighbouring floats (IEEE bit stepping).
 __device__ __forceinline__ float next_down(float f) {
@@ -63,12 +79,33 @@ __device__ __forceinline__ uint32_t morton(T x, T y, T z, const float* box) {
          spread10(q(z, box[2], box[5]));
 }
 
-// box[b] = {lo x y z, hi x y z, 0, 0} of the cloud's finite points (float).
+// Cloud boxes: box[b] = {lo x y z, hi x y z, 0, 0} of the cloud's finite points (float).
+// Many blocks per cloud reduce into order-preserving integer encodings with atomic
+// min/max (exact, so the result does not depend on timing), then a finalize pass
+// decodes them.
+__device__ __forceinline__ unsigned ord_enc(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void box_init_kernel(unsigned* __restrict__ ob, int64_t B) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= B * 6) return;
+  ob[i] = (i % 6) < 3 ? ord_enc(__builtin_inff()) : ord_enc(-__builtin_inff());
+}
+
+constexpr int kBoxChunk = 4096;  // points per reducing block
+
 template <typename T>
 __global__ __launch_bounds__(256) void cloud_box_kernel(const T* __restrict__ p, Clouds cl,
-                                                        float* __restrict__ box) {
-  const int64_t b = blockIdx.x;
-  const int64_t s = cl.p_begin(b), e = cl.p_end(b);
+                                                        unsigned* __restrict__ ob) {
+  const int64_t b = blockIdx.y;
+  const int64_t s = cl.p_begin(b) + static_cast<int64_t>(blockIdx.x) * kBoxChunk;
+  const int64_t e = min(cl.p_end(b), s + kBoxChunk);
+  if (s >= e) return;
   float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
   float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
   for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
@@ -81,29 +118,34 @@ __global__ __launch_bounds__(256) void cloud_box_kernel(const T* __restrict__ p,
       }
     }
   }
-  __shared__ float red[6][256];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    red[c][threadIdx.x] = lo[c];
-    red[3 + c][threadIdx.x] = hi[c];
-  }
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+  for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + w]);
-        red[3 + c][threadIdx.x] = fmaxf(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + w]);
-      }
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fminf(lo[c], __shfl_xor(lo[c], off, 64));
+      hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], off, 64));
     }
-    __syncthreads();
   }
-  if (threadIdx.x < 8) {
-    float v = 0.f;
-    if (threadIdx.x < 6) v = red[threadIdx.x][0];
-    if (threadIdx.x < 6 && !__builtin_isfinite(v)) v = 0.f;  // empty / all non-finite
-    box[b * 8 + threadIdx.x] = v;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      atomicMin(ob + b * 6 + c, ord_enc(lo[c]));
+      atomicMax(ob + b * 6 + 3 + c, ord_enc(hi[c]));
+    }
   }
+}
+
+__global__ void box_final_kernel(const unsigned* __restrict__ ob, int64_t B, float* __restrict__ box) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= B * 8) return;
+  const int64_t b = i / 8;
+  const int c = static_cast<int>(i % 8);
+  float v = 0.f;
+  if (c < 6) {
+    v = ord_dec(ob[b * 6 + c]);
+    if (!__builtin_isfinite(v)) v = 0.f;  // cloud without finite points
+  }
+  box[i] = v;
 }
 
 // keys[i] = (cloud << 32) | morton(row i); rows are query rows (QUERY) or point rows.
@@ -269,13 +311,40 @@ __device__ __forceinline__ int rec_index(T v) {
   else return static_cast<int>(__double_as_longlong(v));
 }
 
-// grid.x = query tiles of 64 sorted queries (ragged: via q tile prefix), grid.y = clouds.
+// Lower bound of every key of a box's points for this lane's query, in the key's
+// precision (the box holds the exact coordinates, so each difference rounds relative
+// to itself), and whether the lane still needs the box.
+template <typename T>
+__device__ __forceinline__ bool box_needed(const float* __restrict__ h, T qx, T qy, T qz, T thr,
+                                           bool valid) {
+  const T mu = static_cast<T>(h[6]);
+  const T dx = fmax(fmax(static_cast<T>(h[0]) - qx, qx - static_cast<T>(h[3])), T(0));
+  const T dy = fmax(fmax(static_cast<T>(h[1]) - qy, qy - static_cast<T>(h[4])), T(0));
+  const T dz = fmax(fmax(static_cast<T>(h[2]) - qz, qz - static_cast<T>(h[5])), T(0));
+  const T lb = mu * (dx * dx + dy * dy + dz * dz);
+  return valid & ((mu < T(0)) | !(lb > thr));
+}
+
+// Outward walk over [lo, hi) from c: c, c+1, c-1, c+2, c-2, ... (each index once).
+struct Outward {
+  int64_t up, down, lo, hi;
+  __device__ Outward(int64_t lo_, int64_t hi_, int64_t c) : up(c), down(c - 1), lo(lo_), hi(hi_) {}
+  __device__ __forceinline__ int64_t next(int64_t step) {
+    const bool take_up = (up < hi) && ((step & 1) == 0 || down < lo);
+    return take_up ? up++ : down--;
+  }
+};
+
+// grid.x = query tiles of 64 sorted queries (ragged: via the query tile prefix),
+// grid.y = clouds (dense).  Two-level walk: super-tiles of kSuper tiles, then tiles.
 template <typename T, int KMAX, int KEY>
 __global__ __launch_bounds__(64) void knn_pruned_kernel(
     const T* __restrict__ q, Clouds cl, const int64_t* __restrict__ qtiles,
     const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys,
-    const int64_t* __restrict__ ptiles, const T* __restrict__ rec, const float* __restrict__ hdr,
-    const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists, int64_t* __restrict__ idxs) {
+    const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
+    const T* __restrict__ rec, const float* __restrict__ hdr, const float* __restrict__ shdr,
+    const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
+    unsigned long long* __restrict__ stats) {
   const int lane = threadIdx.x;
   int64_t b, q0;
   if (qtiles) {
@@ -311,14 +380,18 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     id[s] = kTileSentinel;
   }
 
-  // this cloud's tiles [tb, te)
-  int64_t tb, te;
+  // this cloud's tiles [tb, te) and super-tiles [sb, se)
+  int64_t tb, te, sb, se;
   if (cl.p_off) {
     tb = ptiles[b];
     te = ptiles[b + 1];
+    sb = stiles[b];
+    se = stiles[b + 1];
   } else {
     tb = b * ptiles[1];
     te = tb + ptiles[1];
+    sb = b * stiles[1];
+    se = sb + stiles[1];
   }
   // seed: last tile whose first code <= the wave's first query code (scalar search)
   const uint32_t code0 = static_cast<uint32_t>(qkeys[q0] & 0xffffffffu);
@@ -327,33 +400,43 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     const int64_t mid = (lo + hi + 1) >> 1;
     if (__float_as_uint(hdr[mid * 8 + 7]) <= code0) lo = mid; else hi = mid - 1;
   }
-  const int64_t seed = __builtin_amdgcn_readfirstlane(static_cast<int>(lo - tb)) + tb;
-  const int64_t ntiles = te - tb;
+  const int64_t seed_t = lo;
+  const int64_t seed_s = sb + (seed_t - tb) / kSuper;
 
-  // outward walk: seed, seed+1, seed-1, seed+2, ...
-  int64_t up = seed, down = seed - 1;
-  for (int64_t step = 0; step < ntiles; ++step) {
-    int64_t t;
-    const bool take_up = (up < te) && ((step & 1) == 0 || down < tb);
-    if (take_up) t = up++;
-    else t = down--;
-    // lower bound of every key in the tile, in the key's precision: the box holds the
-    // exact coordinates, so each difference is rounded relative to itself
-    const float* h = hdr + t * 8;
-    const T mu = static_cast<T>(h[6]);
-    const T dx = fmax(fmax(static_cast<T>(h[0]) - qx, qx - static_cast<T>(h[3])), T(0));
-    const T dy = fmax(fmax(static_cast<T>(h[1]) - qy, qy - static_cast<T>(h[4])), T(0));
-    const T dz = fmax(fmax(static_cast<T>(h[2]) - qz, qz - static_cast<T>(h[5])), T(0));
-    const T lb = mu * (dx * dx + dy * dy + dz * dz);
-    const bool need = valid & ((mu < T(0)) | !(lb > key[KMAX - 1]));
-    if (!__any(need)) continue;
-    const int cnt = tcnt[t];
-    const T* __restrict__ base = rec + t * (kTile * 16);
-    for (int j = 0; j < cnt; ++j) {
-      const T* r = base + j * 16;
-      const T c = pair_key<T, KEY>(qx, qy, qz, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7],
-                                   r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
-      if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
+  unsigned n_super = 0, n_tile = 0, n_scan = 0, n_pts = 0;
+  Outward so(sb, se, seed_s);
+  for (int64_t sstep = 0; sstep < se - sb; ++sstep) {
+    const int64_t s = so.next(sstep);
+    ++n_super;
+    if (!__any(box_needed<T>(shdr + s * 8, qx, qy, qz, key[KMAX - 1], valid))) continue;
+    const int64_t t0 = tb + (s - sb) * kSuper;
+    const int64_t t1 = min(te, t0 + kSuper);
+    Outward to(t0, t1, s == seed_s ? seed_t : t0);
+    for (int64_t tstep = 0; tstep < t1 - t0; ++tstep) {
+      const int64_t t = to.next(tstep);
+      ++n_tile;
+      if (!__any(box_needed<T>(hdr + t * 8, qx, qy, qz, key[KMAX - 1], valid))) continue;
+      ++n_scan;
+      const int cnt = tcnt[t];
+      n_pts += cnt;
+      const T* __restrict__ base = rec + t * (kTile * 16);
+      int j = 0;
+      for (; j + 2 <= cnt; j += 2) {  // two records per trip: both loads issue together
+        const T* r = base + j * 16;
+        const T* r2 = r + 16;
+        const T c = pair_key<T, KEY>(qx, qy, qz, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7],
+                                     r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
+        const T c2 = pair_key<T, KEY>(qx, qy, qz, r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6],
+                                      r2[7], r2[8], r2[9], r2[10], r2[11], r2[12], r2[13], r2[14]);
+        if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
+        if (c2 <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c2, rec_index(r2[15]));
+      }
+      if (j < cnt) {
+        const T* r = base + j * 16;
+        const T c = pair_key<T, KEY>(qx, qy, qz, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7],
+                                     r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
+        if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
+      }
     }
   }
 
@@ -366,39 +449,91 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
       }
     }
   }
+  if (stats && lane == 0) {
+    atomicAdd(stats + 0, static_cast<unsigned long long>(n_super));
+    atomicAdd(stats + 1, static_cast<unsigned long long>(n_tile));
+    atomicAdd(stats + 2, static_cast<unsigned long long>(n_scan));
+    atomicAdd(stats + 3, static_cast<unsigned long long>(n_pts));
+    atomicAdd(stats + 4, 1ull);
+  }
+}
+
+// Super-tile headers: box union and min mu (negative if any tile is "always scan")
+// of kSuper consecutive tiles of one cloud.  One thread per super-tile.
+__global__ void super_build_kernel(Clouds cl, const int64_t* __restrict__ ptiles,
+                                   const int64_t* __restrict__ stiles, int64_t nsuper,
+                                   const float* __restrict__ hdr, float* __restrict__ shdr) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nsuper) return;
+  int64_t b = 0, tb, te, sb;
+  if (cl.p_off) {
+    int64_t lo = 0, hi = cl.B - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (stiles[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    b = lo;
+    tb = ptiles[b];
+    te = ptiles[b + 1];
+    sb = stiles[b];
+  } else {
+    b = s / stiles[1];
+    tb = b * ptiles[1];
+    te = tb + ptiles[1];
+    sb = b * stiles[1];
+  }
+  const int64_t t0 = tb + (s - sb) * kSuper;
+  const int64_t t1 = min(te, t0 + kSuper);
+  float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+  float mu = __builtin_inff();
+  bool always = false;
+  for (int64_t t = t0; t < t1; ++t) {
+    const float* h = hdr + t * 8;
+    for (int c = 0; c < 3; ++c) {
+      lo3[c] = fminf(lo3[c], h[c]);
+      hi3[c] = fmaxf(hi3[c], h[3 + c]);
+    }
+    if (h[6] < 0.f) always = true;
+    mu = fminf(mu, h[6]);
+  }
+  float* o = shdr + s * 8;
+  for (int c = 0; c < 3; ++c) {
+    o[c] = lo3[c];
+    o[3 + c] = hi3[c];
+  }
+  o[6] = always ? -1.f : mu;
+  o[7] = 0.f;
 }
 
 template <typename T, int KMAX, int KEY>
-static hipError_t launch_pruned_k(const PrunedLaunch& L, const uint64_t* qkeys, const uint32_t* qperm,
-                                  const int64_t* ptp, const int64_t* qtp, const T* rec,
-                                  const float* hdr, const int32_t* tcnt, hipStream_t st) {
+static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
   dim3 grid;
   if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles), 1);
   else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64), static_cast<unsigned>(L.cl.B));
   hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY>), grid, dim3(64), 0, st, static_cast<const T*>(L.q),
-                     L.cl, L.cl.q_off ? qtp : nullptr, qperm, qkeys, ptp, rec, hdr, tcnt,
-                     static_cast<int>(L.K), static_cast<T*>(L.dists), L.idxs);
+                     L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp, S.stp,
+                     static_cast<const T*>(S.rec), S.hdr, S.shdr, S.tcnt, static_cast<int>(L.K),
+                     static_cast<T*>(L.dists), L.idxs, L.stats);
   return hipGetLastError();
 }
 
 template <typename T, int KEY>
-static hipError_t launch_pruned_topk(const PrunedLaunch& L, const uint64_t* qkeys, const uint32_t* qperm,
-                                     const int64_t* ptp, const int64_t* qtp, const T* rec,
-                                     const float* hdr, const int32_t* tcnt, hipStream_t st) {
-  if (L.K <= 1) return launch_pruned_k<T, 1, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
-  if (L.K <= 4) return launch_pruned_k<T, 4, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
-  if (L.K <= 8) return launch_pruned_k<T, 8, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
-  if (L.K <= 16) return launch_pruned_k<T, 16, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
-  if (L.K <= 32) return launch_pruned_k<T, 32, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
-  return launch_pruned_k<T, 64, KEY>(L, qkeys, qperm, ptp, qtp, rec, hdr, tcnt, st);
+static hipError_t launch_pruned_topk(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+  if (L.K <= 1) return launch_pruned_k<T, 1, KEY>(L, S, st);
+  if (L.K <= 4) return launch_pruned_k<T, 4, KEY>(L, S, st);
+  if (L.K <= 8) return launch_pruned_k<T, 8, KEY>(L, S, st);
+  if (L.K <= 16) return launch_pruned_k<T, 16, KEY>(L, S, st);
+  if (L.K <= 32) return launch_pruned_k<T, 32, KEY>(L, S, st);
+  return launch_pruned_k<T, 64, KEY>(L, S, st);
 }
 
-// ------------------------------------------------------------------ host side
-static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
-
 static int key_bits64(int64_t B) {
-  int bits = 32;
-  while ((int64_t(1) << (bits - 32)) < B) ++bits;
+  int bits = 30;  // 30-bit Morton codes below the cloud id
+  if (B > 1) {
+    bits = 32;
+    while ((int64_t(1) << (bits - 32)) < B) ++bits;
+  }
   return bits;
 }
 
@@ -410,21 +545,61 @@ static hipError_t sort_bytes(int64_t n, int bits, size_t* out) {
                                    nullptr);
 }
 
-// Workspace layout (each piece 256-B aligned):
-//   box[B*8] f32 | pkeys, pkeys_sorted [Prows] u64 | pperm [Prows] u32 |
-//   qkeys, qkeys_sorted [Qrows] u64 | qperm [Qrows] u32 | ptile_prefix, qtile_prefix [B+1] i64 |
-//   rec [T*64*16] | hdr [T*8] f32 | tcnt [T] i32 | sort temp (max of both sorts)
-hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes) {
+static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Workspace carve (each piece 256-B aligned).
+struct Carver {
+  char* base;
+  size_t used = 0;
+  char* take(size_t n) {
+    char* p = base ? base + used : nullptr;
+    used += al(n);
+    return p;
+  }
+};
+
+static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
+  PrunedState S{};
+  Carver c{static_cast<char*>(L.ws)};
+  const size_t el = L.dtype == AKNN_F64 ? 8 : 4;
+  S.box = reinterpret_cast<float*>(c.take(L.cl.B * 8 * 4));
+  S.obox = reinterpret_cast<unsigned*>(c.take(L.cl.B * 6 * 4));
+  S.pkeys = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
+  S.pkeys_s = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
+  S.pperm = reinterpret_cast<uint32_t*>(c.take(L.prows * 4));
+  S.qkeys = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
+  S.qkeys_s = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
+  S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
+  S.ptp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
+  S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
+  S.stp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
+  S.rec = c.take(L.ntiles * kTile * 16 * el);
+  S.hdr = reinterpret_cast<float*>(c.take(L.ntiles * 8 * 4));
+  S.shdr = reinterpret_cast<float*>(c.take(L.nsuper * 8 * 4));
+  S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
+  S.tmp = c.take(tmp_bytes);
+  S.tmp_bytes = tmp_bytes;
+  S.total = c.used;
+  return S;
+}
+
+static hipError_t sort_temp(const PrunedLaunch& L, size_t* bytes) {
   const int bits = key_bits64(L.cl.B);
   size_t sp = 0, sq = 0;
   hipError_t e = sort_bytes(L.prows, bits, &sp);
   if (e != hipSuccess) return e;
   e = sort_bytes(L.qrows, bits, &sq);
+  *bytes = sp > sq ? sp : sq;
+  return e;
+}
+
+hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes) {
+  size_t tmp = 0;
+  const hipError_t e = sort_temp(L, &tmp);
   if (e != hipSuccess) return e;
-  const size_t el = L.dtype == AKNN_F64 ? 8 : 4;
-  *bytes = al(L.cl.B * 8 * 4) + 2 * al(L.prows * 8) + al(L.prows * 4) + 2 * al(L.qrows * 8) +
-           al(L.qrows * 4) + 2 * al((L.cl.B + 1) * 8) + al(L.ntiles * kTile * 16 * el) +
-           al(L.ntiles * 8 * 4) + al(L.ntiles * 4) + al(sp > sq ? sp : sq);
+  PrunedLaunch probe = L;
+  probe.ws = nullptr;
+  *bytes = carve(probe, tmp).total;
   return hipSuccess;
 }
 
@@ -439,60 +614,38 @@ __global__ void tile_prefix_kernel2(const int64_t* __restrict__ off, int64_t B, 
   out[B] = acc;
 }
 
-__global__ void dense_prefix_kernel(int64_t per_cloud_tiles, int64_t* __restrict__ out) {
+// super-tile prefix from the tile prefix (ragged)
+__global__ void super_prefix_kernel(const int64_t* __restrict__ ptp, int64_t B, int64_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t acc = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    out[b] = acc;
+    acc += (ptp[b + 1] - ptp[b] + kSuper - 1) / kSuper;
+  }
+  out[B] = acc;
+}
+
+__global__ void dense_prefix_kernel(int64_t per_cloud, int64_t* __restrict__ out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     out[0] = 0;
-    out[1] = per_cloud_tiles;
+    out[1] = per_cloud;
   }
 }
 
-// Carve the workspace (layout above).
-struct PrunedState {
-  float* box;
-  uint64_t *pkeys, *pkeys_s, *qkeys, *qkeys_s;
-  uint32_t *pperm, *qperm;
-  int64_t *ptp, *qtp;
-  void* rec;
-  float* hdr;
-  int32_t* tcnt;
-  void* tmp;
-  size_t tmp_bytes;
-};
-
-static PrunedState carve(const PrunedLaunch& L) {
-  PrunedState S{};
-  char* w = static_cast<char*>(L.ws);
-  auto take = [&](size_t n) {
-    char* p = w;
-    w += al(n);
-    return p;
-  };
-  const size_t el = L.dtype == AKNN_F64 ? 8 : 4;
-  S.box = reinterpret_cast<float*>(take(L.cl.B * 8 * 4));
-  S.pkeys = reinterpret_cast<uint64_t*>(take(L.prows * 8));
-  S.pkeys_s = reinterpret_cast<uint64_t*>(take(L.prows * 8));
-  S.pperm = reinterpret_cast<uint32_t*>(take(L.prows * 4));
-  S.qkeys = reinterpret_cast<uint64_t*>(take(L.qrows * 8));
-  S.qkeys_s = reinterpret_cast<uint64_t*>(take(L.qrows * 8));
-  S.qperm = reinterpret_cast<uint32_t*>(take(L.qrows * 4));
-  S.ptp = reinterpret_cast<int64_t*>(take((L.cl.B + 1) * 8));
-  S.qtp = reinterpret_cast<int64_t*>(take((L.cl.B + 1) * 8));
-  S.rec = take(L.ntiles * kTile * 16 * el);
-  S.hdr = reinterpret_cast<float*>(take(L.ntiles * 8 * 4));
-  S.tcnt = reinterpret_cast<int32_t*>(take(L.ntiles * 4));
-  S.tmp = w;
-  S.tmp_bytes = L.ws_bytes - static_cast<size_t>(w - static_cast<char*>(L.ws));
-  return S;
-}
-
-// Boxes, Morton codes, both sorts and the tile records/headers.
+// Boxes, Morton codes, both sorts, tile records/headers and super-tile headers.
 template <typename T, int KEY>
 static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
   const int bits = key_bits64(L.cl.B);
   const T* q = static_cast<const T*>(L.q);
   const T* p = static_cast<const T*>(L.p);
-  hipLaunchKernelGGL((cloud_box_kernel<T>), dim3(static_cast<unsigned>(L.cl.B)), dim3(256), 0, st, p, L.cl, S.box);
   const int bs = 256;
+  hipLaunchKernelGGL(box_init_kernel, dim3(static_cast<unsigned>((L.cl.B * 6 + bs - 1) / bs)), dim3(bs), 0, st,
+                     S.obox, L.cl.B);
+  const int64_t chunks = (L.max_points + kBoxChunk - 1) / kBoxChunk;
+  hipLaunchKernelGGL((cloud_box_kernel<T>), dim3(static_cast<unsigned>(chunks > 0 ? chunks : 1),
+                     static_cast<unsigned>(L.cl.B)), dim3(256), 0, st, p, L.cl, S.obox);
+  hipLaunchKernelGGL(box_final_kernel, dim3(static_cast<unsigned>((L.cl.B * 8 + bs - 1) / bs)), dim3(bs), 0, st,
+                     S.obox, L.cl.B, S.box);
   hipLaunchKernelGGL((codes_kernel<T, false>), dim3(static_cast<unsigned>((L.prows + bs - 1) / bs)), dim3(bs), 0,
                      st, p, L.cl, L.prows, S.box, S.pkeys);
   hipLaunchKernelGGL((codes_kernel<T, true>), dim3(static_cast<unsigned>((L.qrows + bs - 1) / bs)), dim3(bs), 0,
@@ -511,17 +664,29 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
   if (L.cl.p_off) {
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.p_off, L.cl.B, int64_t(kTile), S.ptp);
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.q_off, L.cl.B, int64_t(64), S.qtp);
+    hipLaunchKernelGGL(super_prefix_kernel, dim3(1), dim3(64), 0, st, S.ptp, L.cl.B, S.stp);
   } else {
-    hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, (L.cl.P + kTile - 1) / kTile, S.ptp);
+    const int64_t per = (L.cl.P + kTile - 1) / kTile;
+    hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, per, S.ptp);
+    hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, (per + kSuper - 1) / kSuper, S.stp);
   }
   hipLaunchKernelGGL((tile_build_kernel<T, KEY>), dim3(static_cast<unsigned>(L.ntiles)), dim3(64), 0, st,
                      p, static_cast<const T*>(L.sigma), static_cast<const T*>(L.R), L.cl, S.ptp, S.pperm,
                      S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.tcnt);
+  hipLaunchKernelGGL(super_build_kernel, dim3(static_cast<unsigned>((L.nsuper + bs - 1) / bs)), dim3(bs), 0, st,
+                     L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr);
   return hipGetLastError();
 }
 
+static PrunedState carve_run(const PrunedLaunch& L) {
+  size_t tmp = 0;
+  (void)sort_temp(L, &tmp);
+  return carve(L, tmp);
+}
+
 hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st) {
-  const PrunedState S = carve(L);
+  const PrunedState S = carve_run(L);
+  if (S.total > L.ws_bytes) return hipErrorInvalidValue;
   if (L.dtype == AKNN_F32)
     return L.key == AKNN_KEY_EUCLIDEAN ? prep_pruned<float, AKNN_KEY_EUCLIDEAN>(L, S, st)
                                        : prep_pruned<float, AKNN_KEY_ANISO>(L, S, st);
@@ -530,19 +695,37 @@ hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st) {
 }
 
 hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st) {
-  const PrunedState S = carve(L);
-  if (L.dtype == AKNN_F32) {
-    const float* rec = static_cast<const float*>(S.rec);
-    return L.key == AKNN_KEY_EUCLIDEAN
-               ? launch_pruned_topk<float, AKNN_KEY_EUCLIDEAN>(L, S.qkeys_s, S.qperm, S.ptp, S.qtp, rec, S.hdr, S.tcnt, st)
-               : launch_pruned_topk<float, AKNN_KEY_ANISO>(L, S.qkeys_s, S.qperm, S.ptp, S.qtp, rec, S.hdr, S.tcnt, st);
-  }
-  const double* rec = static_cast<const double*>(S.rec);
-  return L.key == AKNN_KEY_EUCLIDEAN
-             ? launch_pruned_topk<double, AKNN_KEY_EUCLIDEAN>(L, S.qkeys_s, S.qperm, S.ptp, S.qtp, rec, S.hdr, S.tcnt, st)
-             : launch_pruned_topk<double, AKNN_KEY_ANISO>(L, S.qkeys_s, S.qperm, S.ptp, S.qtp, rec, S.hdr, S.tcnt, st);
+  const PrunedState S = carve_run(L);
+  if (L.dtype == AKNN_F32)
+    return L.key == AKNN_KEY_EUCLIDEAN ? launch_pruned_topk<float, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                       : launch_pruned_topk<float, AKNN_KEY_ANISO>(L, S, st);
+  return L.key == AKNN_KEY_EUCLIDEAN ? launch_pruned_topk<double, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                     : launch_pruned_topk<double, AKNN_KEY_ANISO>(L, S, st);
 }
 
-const uint32_t* pruned_query_order(const PrunedLaunch& L) { return carve(L).qperm; }
+const uint32_t* pruned_query_order(const PrunedLaunch& L) { return carve_run(L).qperm; }
+const uint32_t* pruned_point_order(const PrunedLaunch& L) { return carve_run(L).pperm; }
+
+__global__ void identity_kernel(int32_t* __restrict__ out, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<int32_t>(i);
+}
+
+__global__ void copy_u32_kernel(const uint32_t* __restrict__ src, int32_t* __restrict__ dst, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = static_cast<int32_t>(src[i]);
+}
+
+hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(identity_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_u32_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
+}
 
 }  // namespace aknn

@@ -26,6 +26,7 @@ __all__ = [
     "knn_aniso",
     "build_csr",
     "make_dims",
+    "new_cache",
 ]
 
 _DTYPES = {torch.float32: _lib.AKNN_F32, torch.float64: _lib.AKNN_F64}
@@ -116,6 +117,8 @@ def _tune(tune) -> Optional[_lib.Tune]:
     t.queries_per_lane = int(tune.get("queries_per_lane", 0))
     t.point_split = int(tune.get("point_split", 0))
     t.search = _lib.SEARCH[tune.get("search", "auto")]
+    st = tune.get("stats")  # optional int64 device tensor of 5 counters
+    t.stats = st.data_ptr() if st is not None else None
     return t
 
 
@@ -123,13 +126,34 @@ def _tp(t: Optional[_lib.Tune]):
     return ctypes.byref(t) if t is not None else None
 
 
+def new_cache(B, Q, P, dtype, device):
+    """Tensors of an ``aknn_cache`` for a [B,Q] x [B,P] batch: spatial orders of the
+    query and point rows and packed point records (filled by the forward)."""
+    return (torch.empty(B * Q, dtype=torch.int32, device=device),
+            torch.empty(B * P, dtype=torch.int32, device=device),
+            torch.empty(B * P * 16, dtype=dtype, device=device))
+
+
+def _cache_struct(cache):
+    if cache is None:
+        return None
+    s = _lib.Cache()
+    s.query_order, s.point_order, s.point_records = (t.data_ptr() if t is not None else None for t in cache)
+    return s
+
+
+def _cp(s):
+    return ctypes.byref(s) if s is not None else None
+
+
 # ----------------------------------------------------------------------------- forward
-def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tune=None):
+def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tune=None, cache=None):
     """Neighbour search + aggregation -> ``[f_out, w_out, dists, idxs]``.
 
     Reference: ``knn_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:26-47,
     csrc/forward.cu:286-338).  ``f_out`` [B,Q,F], ``w_out``/``dists`` [B,Q,K],
-    ``idxs`` [B,Q,K] int64; neighbours ordered by (key, point index).
+    ``idxs`` [B,Q,K] int64; neighbours ordered by (key, point index).  ``cache`` (from
+    :func:`new_cache`) is filled for the backward passes; results never depend on it.
     """
     for t, n in zip((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R")):
         _check_input(t, n)
@@ -148,11 +172,12 @@ def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tun
     ws_bytes = lib.aknn_forward_workspace_size(ctypes.byref(dims))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     tn = _tune(tune)
+    cs = _cache_struct(cache)
     with torch.cuda.device(dev):
         _lib.check(lib.aknn_forward(ctypes.byref(dims), _tp(tn), q.data_ptr(), p.data_ptr(),
                                     f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
                                     f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
-                                    idxs.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
+                                    idxs.data_ptr(), ws.data_ptr(), ws_bytes, _cp(cs), _stream(dev)))
     return [f_out, w_out, dists, idxs]
 
 
@@ -270,7 +295,7 @@ def _bwd_common(q, p, f, sigma, R, k_idxs, grads, gnames):
 
 
 def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, k_idxs, *,
-                                 key: str = "aniso", csr=None, needs=None):
+                                 key: str = "aniso", csr=None, needs=None, cache=None):
     """First-order gradients -> ``[dq, dp, df, dsigma, dR]``.
 
     Reference: ``knn_aggregate_aniso_backward`` (csrc/knn_aggregate.cpp:101-132,
@@ -301,14 +326,15 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     with torch.cuda.device(dev):
         _lib.check(lib.aknn_backward(ctypes.byref(dims), mask, grad_f_out.data_ptr(), q.data_ptr(),
                                      p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
-                                     k_idxs.data_ptr(), P_(off), P_(ent), P_(outs["dq"]),
+                                     k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
+                                     P_(outs["dq"]),
                                      P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
                                      P_(outs["dR"]), _stream(dev)))
     return [outs[n] for n in names]
 
 
 def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out,
-                                     k_idxs, *, key: str = "aniso", csr=None, needs=None):
+                                     k_idxs, *, key: str = "aniso", csr=None, needs=None, cache=None):
     """Second-order gradients of ``<grad_grad_q, dq>`` ->
     ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
 
@@ -342,7 +368,8 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         _lib.check(lib.aknn_backward2(ctypes.byref(dims), mask, grad_grad_q.data_ptr(),
                                       grad_f_out.data_ptr(), q.data_ptr(), p.data_ptr(),
                                       f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
-                                      k_idxs.data_ptr(), P_(off), P_(ent), P_(outs["dgf"]),
+                                      k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
+                                      P_(outs["dgf"]),
                                       P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]),
                                       P_(outs["dsigma"]), P_(outs["dR"]), _stream(dev)))
     return [outs[n] for n in names]

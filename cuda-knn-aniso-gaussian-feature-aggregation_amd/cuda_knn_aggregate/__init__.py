@@ -65,17 +65,24 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, q, p, f, sigma, R, K, key="aniso"):
-        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key)
+        # The cache (spatial row orders, packed point records) only speeds up the
+        # backward kernels; it is filled when a backward can follow.
+        cache = None
+        if any(ctx.needs_input_grad[:5]) and q.is_cuda and p.dim() == 3 and q.dim() == 3:
+            cache = _C.new_cache(q.shape[0], q.shape[1], p.shape[1], q.dtype, q.device)
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key, cache=cache)
         # Like the reference (__init__.py:89); the backward kernels recompute the
         # weights from idxs and never read w_out.
-        ctx.save_for_backward(q, p, f, sigma, R, w_out, idxs)
+        ctx.save_for_backward(q, p, f, sigma, R, w_out, idxs, *(cache or ()))
         ctx.key = key
+        ctx.has_cache = cache is not None
         ctx.set_materialize_grads(False)
         return f_out, w_out, dists, idxs
 
     @staticmethod
     def backward(ctx, grad_f_out, grad_w_out, grad_dists, grad_idxs):
-        q, p, f, sigma, R, w_out, idxs = ctx.saved_tensors
+        q, p, f, sigma, R, w_out, idxs = ctx.saved_tensors[:7]
+        cache = tuple(ctx.saved_tensors[7:10]) if getattr(ctx, "has_cache", False) else None
         # Only grad_f_out enters (grad_w_out and grad_dists are ignored, as in the
         # reference, csrc/backward.cu:93-94): no grad on f_out means no gradient at all.
         if grad_f_out is None:
@@ -97,6 +104,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
             idxs,
             ctx.key,
             needs,
+            cache,
         )
         return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None
 
@@ -111,6 +119,7 @@ class KNNPrecomputedAnisotropicAggregate(KNNAnisotropicAggregate):
         f_out, w_out, dists = _C.knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, key=key)
         ctx.save_for_backward(q, p, f, sigma, R, w_out, knn_idxs)
         ctx.key = key
+        ctx.has_cache = False
         ctx.set_materialize_grads(False)
         return f_out, w_out, dists
 
@@ -130,18 +139,22 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key="aniso",
-                needs=None):
+                needs=None, cache=None):
         if needs is None:
             needs = (True,) * 5
         P = p.shape[1]
         csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) else None
         dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward(
-            grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs)
+            grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs,
+            cache=cache)
         saved = [grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs]
         if csr is not None:
             saved += list(csr)
+        if cache is not None:
+            saved += list(cache)
         ctx.save_for_backward(*saved)
         ctx.has_csr = csr is not None
+        ctx.has_cache = cache is not None
         ctx.key = key
         return dq, dp, df, dsigma, dR
 
@@ -149,12 +162,14 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
     def backward(ctx, grad_grad_q, grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R):
         saved = ctx.saved_tensors
         grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs = saved[:9]
-        csr = (saved[9], saved[10]) if ctx.has_csr else None
+        rest = list(saved[9:])
+        csr = (rest.pop(0), rest.pop(0)) if ctx.has_csr else None
+        cache = tuple(rest[:3]) if ctx.has_cache else None
         ni = ctx.needs_input_grad
         # outputs: d grad_f_out, dq, dp, df, dsigma, dR
         needs = (ni[0], ni[2], ni[3], ni[4], ni[5], ni[6])
         if grad_grad_q is None or not any(needs):
-            return (None,) * 11
+            return (None,) * 12
         if csr is None and _point_needs(needs[2:]):
             csr = _C.build_csr(idxs, p.shape[1], ctx.key)
         d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
@@ -171,5 +186,6 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
             key=ctx.key,
             csr=csr,
             needs=needs,
+            cache=cache,
         )
-        return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None
+        return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None, None

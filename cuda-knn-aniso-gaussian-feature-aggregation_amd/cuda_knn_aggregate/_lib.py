@@ -70,11 +70,18 @@ class Dims(ctypes.Structure):
     ]
 
 
+class Cache(ctypes.Structure):
+    """``aknn_cache`` (include/aknn.h)."""
+
+    _fields_ = [("query_order", ctypes.c_void_p), ("point_order", ctypes.c_void_p),
+                ("point_records", ctypes.c_void_p)]
+
+
 class Tune(ctypes.Structure):
     """``aknn_tune`` (include/aknn.h)."""
 
     _fields_ = [("queries_per_lane", ctypes.c_int32), ("point_split", ctypes.c_int32),
-                ("search", ctypes.c_int32)]
+                ("search", ctypes.c_int32), ("reserved", ctypes.c_int32), ("stats", ctypes.c_void_p)]
 
 
 SEARCH = {"auto": 0, "brute": 1, "tiles": 2}
@@ -86,19 +93,20 @@ _lib = None
 _P = ctypes.c_void_p
 _DIMS = ctypes.POINTER(Dims)
 _TUNE = ctypes.POINTER(Tune)
+_CACHE = ctypes.POINTER(Cache)
 _SIGS = {
     "aknn_abi_version": (ctypes.c_int, []),
     "aknn_last_error": (ctypes.c_char_p, []),
     "aknn_trace_enable": (None, [ctypes.c_int]),
     "aknn_trace_collect": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
     "aknn_forward_workspace_size": (ctypes.c_size_t, [_DIMS]),
-    "aknn_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 9 + [_P, ctypes.c_size_t, _P]),
+    "aknn_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 9 + [_P, ctypes.c_size_t, _CACHE, _P]),
     "aknn_knn": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 6 + [_P, ctypes.c_size_t, _P]),
     "aknn_aggregate": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P] * 3 + [_P]),
     "aknn_csr_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_build_csr": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, ctypes.c_size_t, _P]),
-    "aknn_backward": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 9 + [_P] * 5 + [_P]),
-    "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_P] * 6 + [_P]),
+    "aknn_backward": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 9 + [_CACHE] + [_P] * 5 + [_P]),
+    "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE] + [_P] * 6 + [_P]),
 }
 
 

@@ -94,7 +94,24 @@ typedef struct aknn_tune {
   int32_t queries_per_lane; /* brute: 1, 2 or 4 queries held per lane                  */
   int32_t point_split;      /* brute: 1, 2, 4 or 8 waves splitting one tile's points   */
   int32_t search;           /* AKNN_SEARCH_*                                            */
+  int32_t reserved;
+  void* stats;              /* tiles: optional device uint64[5] the search adds its counts
+                               to (super-tile tests, tile tests, tiles scanned, points
+                               scanned, waves); NULL = off                               */
 } aknn_tune;
+
+/* Optional per-batch cache that aknn_forward fills and the backward passes read (no
+ * counterpart in the reference; for callers that keep state between the passes, like
+ * the autograd Functions).  Each pointer may be NULL.  Results never depend on it: it
+ * only sets the order rows are processed in (spatial orders keep gathered rows in L2)
+ * and lets the backward kernels read point data as packed 16-value records.
+ * Sizes: query_order int32[query rows]; point_order int32[point rows];
+ * point_records dtype[point rows * 16]. */
+typedef struct aknn_cache {
+  int32_t* query_order;
+  int32_t* point_order;
+  void* point_records;
+} aknn_cache;
 
 AKNN_API int aknn_abi_version(void);
 AKNN_API const char* aknn_last_error(void);
@@ -121,7 +138,7 @@ AKNN_API size_t aknn_forward_workspace_size(const aknn_dims* dims);
 AKNN_API int aknn_forward(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
                  const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
                  void* dists, int64_t* idxs, void* workspace, size_t workspace_bytes,
-                 void* stream);
+                 aknn_cache* cache, void* stream);
 
 /* Neighbour search only (idxs + dists).  Used by the aggregation-free paths. */
 AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
@@ -152,7 +169,8 @@ AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t*
 AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
-                  void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream);
+                  const aknn_cache* cache, void* dq, void* dp, void* df, void* dsigma, void* dR,
+                  void* stream);
 
 /* Second-order backward of <grad_grad_q, dq>.  Replaces knn_aggregate_aniso_backward_2nd
  * (csrc/knn_aggregate.cpp:135-172); like the reference only grad_grad_q is an input
@@ -160,8 +178,9 @@ AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* gr
 AKNN_API int aknn_backward2(const aknn_dims* dims, uint32_t needs, const void* grad_grad_q,
                    const void* grad_f_out, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs,
-                   const int32_t* csr_offsets, const int32_t* csr_entries, void* d_grad_f_out,
-                   void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream);
+                   const int32_t* csr_offsets, const int32_t* csr_entries,
+                   const aknn_cache* cache, void* d_grad_f_out, void* dq, void* dp, void* df,
+                   void* dsigma, void* dR, void* stream);
 
 #ifdef __cplusplus
 }

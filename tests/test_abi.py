@@ -46,7 +46,7 @@ def _dims(**kw):
 def test_forward_rejects_bad_dims(kw, code, msg):
     lib = _lib.load()
     d = _dims(**kw)
-    rc = lib.aknn_forward(ctypes.byref(d), None, *([None] * 9), None, 0, None)
+    rc = lib.aknn_forward(ctypes.byref(d), None, *([None] * 9), None, 0, None, None)
     assert rc == code
     assert msg in lib.aknn_last_error().decode()
 
@@ -60,7 +60,7 @@ def test_forward_rejects_small_workspace():
         pytest.skip("no device for the workspace size query")
     assert need >= 2 * 8 * 16 * 4
     fake = ctypes.c_void_p(0x1000)
-    rc = lib.aknn_forward(ctypes.byref(d), None, *([fake] * 9), fake, need - 1, None)
+    rc = lib.aknn_forward(ctypes.byref(d), None, *([fake] * 9), fake, need - 1, None, None)
     assert rc == _lib.AKNN_E_WORKSPACE
 
 
@@ -68,13 +68,13 @@ def test_backward_rejects_missing_outputs_and_csr():
     lib = _lib.load()
     d = _dims()
     fake = ctypes.c_void_p(0x1000)
-    rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DQ, *([fake] * 7), None, None,
+    rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DQ, *([fake] * 7), None, None, None,
                            None, None, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "dq requested" in lib.aknn_last_error().decode()
-    rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DP, *([fake] * 7), None, None,
+    rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DP, *([fake] * 7), None, None, None,
                            None, fake, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "CSR" in lib.aknn_last_error().decode()
-    rc = lib.aknn_backward2(ctypes.byref(d), _lib.NEED_DQ, None, *([fake] * 7), None, None,
+    rc = lib.aknn_backward2(ctypes.byref(d), _lib.NEED_DQ, None, *([fake] * 7), None, None, None,
                             None, fake, None, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "grad_grad_q" in lib.aknn_last_error().decode()
 
@@ -84,7 +84,7 @@ def test_ragged_offsets_validated():
     qh = torch.tensor([0, 5, 3], dtype=torch.int64)      # decreasing
     ph = torch.tensor([0, 4, 8], dtype=torch.int64)
     d = _C.make_dims(2, 3, 8, 4, 2, torch.float32, offsets=(qh, ph, qh, ph))
-    rc = lib.aknn_forward(ctypes.byref(d), None, *([ctypes.c_void_p(0x1000)] * 9), None, 0, None)
+    rc = lib.aknn_forward(ctypes.byref(d), None, *([ctypes.c_void_p(0x1000)] * 9), None, 0, None, None)
     assert rc == _lib.AKNN_E_INVALID and "offsets" in lib.aknn_last_error().decode()
 
 

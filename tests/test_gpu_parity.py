@@ -296,3 +296,29 @@ def test_errors(cuda):
         knn_aggregate_aniso(b.q.double(), b.p, b.f, b.sigma, b.R, 4)
     with pytest.raises(RuntimeError, match="shape"):
         knn_aggregate_aniso(b.q, b.p, b.f, b.sigma[:, :10].contiguous(), b.R, 4)
+
+
+@pytest.mark.parametrize("search", SEARCHES)
+def test_cache_does_not_change_results(cuda, search):
+    """The forward's cache (spatial row orders, packed point records) changes only the
+    processing order of the gather kernels: every output is bit-identical."""
+    b = workloads.clustered_batch(Q=5000, P=4000, K=16, F=32, seed=8, with_grad=True).to(cuda)
+    args = (b.q, b.p, b.f, b.sigma, b.R)
+    cache = _C.new_cache(1, 5000, 4000, torch.float32, cuda)
+    r1 = _C.knn_aggregate_aniso_forward(*args, b.K, tune={"search": search})
+    r2 = _C.knn_aggregate_aniso_forward(*args, b.K, tune={"search": search}, cache=cache)
+    for x, y in zip(r1, r2):
+        assert torch.equal(x, y)
+    qo, po, rec = cache
+    assert torch.equal(qo.sort().values.cpu(), torch.arange(5000, dtype=torch.int32))
+    assert torch.equal(po.sort().values.cpu(), torch.arange(4000, dtype=torch.int32))
+    w_out, idxs = r1[1], r1[3]
+    g1 = _C.knn_aggregate_aniso_backward(b.g, w_out, *args, w_out, idxs)
+    g2 = _C.knn_aggregate_aniso_backward(b.g, w_out, *args, w_out, idxs, cache=cache)
+    for x, y in zip(g1, g2):
+        assert torch.equal(x, y)
+    h = torch.randn_like(b.q)
+    s1 = _C.knn_aggregate_aniso_backward_2nd(h, b.g, w_out, *args, w_out, idxs)
+    s2 = _C.knn_aggregate_aniso_backward_2nd(h, b.g, w_out, *args, w_out, idxs, cache=cache)
+    for x, y in zip(s1, s2):
+        assert torch.equal(x, y)
