@@ -486,15 +486,25 @@ int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets
   return AKNN_OK;
 }
 
+// dq via per-entry contributions when the point kernels run anyway.
+bool contrib_path(const aknn_dims* d, uint32_t needs) {
+  return (needs & AKNN_NEED_DQ) && (needs & AKNN_NEED_POINTS) && point_rows(d) > 0 && query_rows(d) > 0;
+}
+
 static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, const void* g,
                            const void* q, const void* p, const void* f, const void* sigma,
                            const void* R, const int64_t* idxs, const int32_t* csr_off,
-                           const int32_t* csr_ent, const aknn_cache* cache, void* dgf, void* dq,
-                           void* dp, void* df, void* dsigma, void* dR, void* stream) {
+                           const int32_t* csr_ent, const aknn_cache* cache, void* ws,
+                           size_t ws_bytes, void* dgf, void* dq, void* dp, void* df, void* dsigma,
+                           void* dR, void* stream) {
   int rc = check_dims(d, true);
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool second = gg != nullptr;
+  const bool via_contrib = contrib_path(d, needs);
+  if (via_contrib && (!ws || ws_bytes < aknn_backward_workspace_size(d, needs)))
+    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
+                aknn_backward_workspace_size(d, needs));
   if (!g || !q || !p || !f || !sigma || !R || (query_rows(d) > 0 && !idxs))
     return fail(AKNN_E_INVALID, "null input pointer");
   if ((needs & AKNN_NEED_DQ) && !dq) return fail(AKNN_E_INVALID, "dq requested but NULL");
@@ -503,7 +513,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
   if ((needs & AKNN_NEED_DF) && !df) return fail(AKNN_E_INVALID, "df requested but NULL");
   if ((needs & AKNN_NEED_DSIGMA) && !dsigma) return fail(AKNN_E_INVALID, "dsigma requested but NULL");
   if ((needs & AKNN_NEED_DR) && !dR) return fail(AKNN_E_INVALID, "dR requested but NULL");
-  const bool want_q = (needs & AKNN_NEED_DQ) || (second && (needs & AKNN_NEED_DGRAD_F_OUT));
+  const bool want_q = ((needs & AKNN_NEED_DQ) && !via_contrib) || (second && (needs & AKNN_NEED_DGRAD_F_OUT));
   if (want_q && query_rows(d) > 0) {
     aknn::QueryLaunch L{};
     L.dtype = d->dtype;
@@ -516,7 +526,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.idxs = idxs;
     L.grad_f_out = g;
     L.grad_grad_q = gg;
-    L.dq = (needs & AKNN_NEED_DQ) ? dq : nullptr;
+    L.dq = ((needs & AKNN_NEED_DQ) && !via_contrib) ? dq : nullptr;
     L.d_grad_f_out = (needs & AKNN_NEED_DGRAD_F_OUT) ? dgf : nullptr;
     L.order = cache ? cache->query_order : nullptr;
     L.rec = cache ? cache->point_records : nullptr;
@@ -546,35 +556,50 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.dsigma = (needs & AKNN_NEED_DSIGMA) ? dsigma : nullptr;
     L.dR = (needs & AKNN_NEED_DR) ? dR : nullptr;
     L.order = cache ? cache->point_order : nullptr;
+    L.dq_contrib = via_contrib ? ws : nullptr;
     hipError_t e;
     {
       Span sp(second ? "point_grad2" : "point_grad", st);
       e = aknn::launch_point_grad(L, st);
     }
     if (e != hipSuccess) return hip_fail(e, second ? "point_grad2" : "point_grad");
+    if (via_contrib) {
+      {
+        Span sp(second ? "dq_reduce2" : "dq_reduce", st);
+        e = aknn::launch_contrib_reduce(ws, query_rows(d), d->K, dq, d->dtype, st);
+      }
+      if (e != hipSuccess) return hip_fail(e, "dq_reduce");
+    }
   }
   return AKNN_OK;
+}
+
+size_t aknn_backward_workspace_size(const aknn_dims* d, uint32_t needs) {
+  if (check_dims(d, false) != AKNN_OK) return 0;
+  if (!contrib_path(d, needs)) return 0;
+  return align256(static_cast<size_t>(query_rows(d)) * d->K * 3 * elem_size(d));
 }
 
 int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
-                  const aknn_cache* cache, void* dq, void* dp, void* df, void* dsigma, void* dR,
-                  void* stream) {
+                  const aknn_cache* cache, void* ws, size_t ws_bytes, void* dq, void* dp, void* df,
+                  void* dsigma, void* dR, void* stream) {
   return backward_common(d, needs & ~AKNN_NEED_DGRAD_F_OUT, nullptr, grad_f_out, q, p, f, sigma, R,
-                         idxs, csr_offsets, csr_entries, cache, nullptr, dq, dp, df, dsigma, dR,
-                         stream);
+                         idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, nullptr, dq, dp, df,
+                         dsigma, dR, stream);
 }
 
 int aknn_backward2(const aknn_dims* d, uint32_t needs, const void* grad_grad_q,
                    const void* grad_f_out, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs,
                    const int32_t* csr_offsets, const int32_t* csr_entries,
-                   const aknn_cache* cache, void* d_grad_f_out, void* dq, void* dp, void* df,
-                   void* dsigma, void* dR, void* stream) {
+                   const aknn_cache* cache, void* ws, size_t ws_bytes, void* d_grad_f_out,
+                   void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
   if (!grad_grad_q) return fail(AKNN_E_INVALID, "grad_grad_q is NULL");
   return backward_common(d, needs, grad_grad_q, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets,
-                         csr_entries, cache, d_grad_f_out, dq, dp, df, dsigma, dR, stream);
+                         csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp, df, dsigma, dR,
+                         stream);
 }
 
 }  // extern "C"

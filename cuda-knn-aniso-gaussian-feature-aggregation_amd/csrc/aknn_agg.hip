@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   const int K = static_cast<int>(L.K);
 
   const int r = threadIdx.x & 15;
-  const int64_t gi = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gi = blk * 16 + (threadIdx.x >> 4);
   const bool row_ok = gi < L.rows;
   const int64_t gc = row_ok ? gi : L.rows - 1;  // keep the row's lanes active
   // spatial processing order (when given) keeps the gathered rows of neighbouring
@@ -96,6 +97,14 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   // feature slice of this lane: chunk c covers [(c*16 + r)*VEC, +VEC)
   T acc[CH][VEC];
   T gsl[CH][VEC];
+  int64_t fofs[CH];  // clamped into the row for lanes past F
+  bool fok[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+    fok[c] = ft < F;
+    fofs[c] = fok[c] ? ft : 0;
+  }
 #pragma unroll
   for (int c = 0; c < CH; ++c)
 #pragma unroll
@@ -119,6 +128,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     hz = gg[row * 3 + 2];
   }
   T dqx = T(0), dqy = T(0), dqz = T(0);
+  const bool want_dq = L.dq != nullptr;  // uniform: otherwise only d grad_f_out is made
 
   for (int kb = 0; kb < K; kb += 16) {
     const int k = kb + r;
@@ -153,29 +163,36 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
       }
     }
 
+    // All 16 row gathers of the batch are issued back to back: no per-lane branch in
+    // this loop (lanes past F read a clamped in-row address and mask the value).
+    int nj[16];
+    T cj[16];
+    static_for<0, 16>([&](auto J) {
+      nj[J.value] = row_get<J.value>(n);
+      cj[J.value] = row_get<J.value>(coef);
+    });
+    VecT<T, VEC> fv[16][CH];
+    static_for<0, 16>([&](auto J) {
+      const T* __restrict__ frow = f + (pb + nj[J.value]) * F;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) fv[J.value][c] = *reinterpret_cast<const VecT<T, VEC>*>(frow + fofs[c]);
+    });
     T part[16];
     static_for<0, 16>([&](auto J) {
-      const int nj = row_get<J.value>(n);
-      const T cj = row_get<J.value>(coef);
-      const T* __restrict__ frow = f + (pb + nj) * F;
       T dot = T(0);
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
-        if (ft < F) {
-          T fv[VEC];
-          vload<T, VEC>(fv, frow + ft);
 #pragma unroll
-          for (int i = 0; i < VEC; ++i) {
-            if constexpr (MODE != kBwd1) acc[c][i] += cj * fv[i];
-            if constexpr (MODE != kFwd) dot += gsl[c][i] * fv[i];
-          }
+        for (int i = 0; i < VEC; ++i) {
+          const T x = fok[c] ? fv[J.value][c].v[i] : T(0);
+          if constexpr (MODE != kBwd1) acc[c][i] += cj[J.value] * x;
+          if constexpr (MODE != kFwd) dot += gsl[c][i] * x;
         }
       }
       part[J.value] = dot;
     });
 
-    if constexpr (MODE != kFwd) {
+    if (MODE != kFwd && want_dq) {
       const T a = row_transpose_sum<T>(part, r);  // a for slot k
       if (have) {
         if constexpr (MODE == kBwd1) {
@@ -202,7 +219,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     }
   }
 
-  if constexpr (MODE != kFwd) {
+  if (MODE != kFwd && want_dq) {
     dqx = row_sum(dqx);
     dqy = row_sum(dqy);
     dqz = row_sum(dqz);

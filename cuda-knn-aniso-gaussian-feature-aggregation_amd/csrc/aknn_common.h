@@ -51,6 +51,18 @@ struct Clouds {
   __device__ __forceinline__ int64_t p_end(int64_t b) const { return p_off ? p_off[b + 1] : (b + 1) * P; }
 };
 
+// ------------------------------------------------------------------ XCD-aware block order
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with its
+// own L2.  Kernels whose neighbouring logical blocks share data (spatially ordered
+// rows) remap the hardware block id so each XCD works through one contiguous range of
+// logical blocks.  A bijection on [0, n) for any n; results never depend on it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t hw, int64_t n) {
+  const int64_t q = n / kXcds, r = n % kXcds;
+  const int64_t x = hw % kXcds, i = hw / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // ------------------------------------------------------------------ key
 // The neighbour key, one rounding per operation in the reference's association
 // order (csrc/forward.cu:85-92 and :154).  Contraction into FMA is disabled here so

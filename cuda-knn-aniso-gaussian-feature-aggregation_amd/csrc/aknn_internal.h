@@ -70,6 +70,7 @@ struct PointLaunch {
   const void* grad_grad_q;  // nullptr for first order
   void *dp, *df, *dsigma, *dR;
   const int32_t* order;     // optional processing order of the point rows (spatial)
+  void* dq_contrib;         // optional [query rows * K * 3]: each entry's share of dq
 };
 
 template <typename T>
@@ -88,6 +89,9 @@ hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st);
 // spatial orders of the query and point rows (valid after the prep, inside the workspace)
 const uint32_t* pruned_query_order(const PrunedLaunch& L);
 const uint32_t* pruned_point_order(const PrunedLaunch& L);
+
+hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, void* dq, int32_t dtype,
+                                 hipStream_t st);
 
 // CSR build
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);

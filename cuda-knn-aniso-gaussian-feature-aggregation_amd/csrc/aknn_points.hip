@@ -65,7 +65,8 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int K = static_cast<int>(L.K);
 
   const int r = threadIdx.x & 15;
-  const int64_t gi = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gi = blk * 16 + (threadIdx.x >> 4);
   const bool row_ok = gi < L.rows;
   const int64_t gc = row_ok ? gi : L.rows - 1;
   // spatial processing order (when given): neighbouring points share queries, so the
@@ -78,9 +79,13 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int e1 = row_ok ? off[row + 1] : e0;  // same trip count for the whole row
 
   T fsl[CH][VEC], dfa[CH][VEC];
+  int64_t fofs[CH];  // this lane's feature offset per chunk, clamped into the row
+  bool fok[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+    fok[c] = ft < F;
+    fofs[c] = fok[c] ? ft : 0;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) {
       dfa[c][i] = T(0);
@@ -116,22 +121,30 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     }
     const int qi32 = static_cast<int>(qrow);
 
+    // All 16 row gathers of the batch are issued back to back: no per-lane branch in
+    // this loop (lanes past F read a clamped in-row address and mask the value).
+    int qj[16];
+    T cj[16];
+    pstatic_for<0, 16>([&](auto J) {
+      qj[J.value] = prow_get<J.value>(qi32);
+      cj[J.value] = prow_get<J.value>(coef);
+    });
+    PVec<T, VEC> xs[16][CH];
+    pstatic_for<0, 16>([&](auto J) {
+      const T* __restrict__ grow = g + static_cast<int64_t>(qj[J.value]) * F;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) xs[J.value][c] = *reinterpret_cast<const PVec<T, VEC>*>(grow + fofs[c]);
+    });
     T part[16];
     pstatic_for<0, 16>([&](auto J) {
-      const int qj = prow_get<J.value>(qi32);
-      const T cj = prow_get<J.value>(coef);
-      const T* __restrict__ grow = g + static_cast<int64_t>(qj) * F;
       T dot = T(0);
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
-        if (ft < F) {
-          const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(grow + ft);
 #pragma unroll
-          for (int i = 0; i < VEC; ++i) {
-            dfa[c][i] += cj * x.v[i];
-            dot += fsl[c][i] * x.v[i];
-          }
+        for (int i = 0; i < VEC; ++i) {
+          const T x = fok[c] ? xs[J.value][c].v[i] : T(0);
+          dfa[c][i] += cj[J.value] * x;
+          dot += fsl[c][i] * x;
         }
       }
       part[J.value] = dot;
@@ -143,10 +156,17 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
       const T d[3] = {geo.d0, geo.d1, geo.d2};
       const T sg[3] = {pt.s0, pt.s1, pt.s2};
       const T aw = a * w;
+      T* __restrict__ cq = static_cast<T*>(L.dq_contrib);
+      const int64_t cslot = static_cast<int64_t>(entry);
       if constexpr (!SECOND) {
         acc[0] += T(2) * aw * geo.v0;
         acc[1] += T(2) * aw * geo.v1;
         acc[2] += T(2) * aw * geo.v2;
+        if (cq) {  // this entry's share of dq (= minus its share of dp)
+          cq[cslot * 3 + 0] = T(-2) * aw * geo.v0;
+          cq[cslot * 3 + 1] = T(-2) * aw * geo.v1;
+          cq[cslot * 3 + 2] = T(-2) * aw * geo.v2;
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           acc[3 + i] += -aw * u[i] * u[i];
@@ -164,9 +184,15 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
         const T mh1 = pt.r[1] * m0 + pt.r[4] * m1 + pt.r[7] * m2;
         const T mh2 = pt.r[2] * m0 + pt.r[5] * m1 + pt.r[8] * m2;
         const T aw2 = T(-2) * aw;
-        acc[0] += aw2 * (T(2) * s * geo.v0 - mh0);
-        acc[1] += aw2 * (T(2) * s * geo.v1 - mh1);
-        acc[2] += aw2 * (T(2) * s * geo.v2 - mh2);
+        const T t0 = T(2) * s * geo.v0 - mh0, t1 = T(2) * s * geo.v1 - mh1, t2 = T(2) * s * geo.v2 - mh2;
+        acc[0] += aw2 * t0;
+        acc[1] += aw2 * t1;
+        acc[2] += aw2 * t2;
+        if (cq) {  // this entry's share of dq: 2 a w (2 s v - M h)
+          cq[cslot * 3 + 0] = T(2) * aw * t0;
+          cq[cslot * 3 + 1] = T(2) * aw * t1;
+          cq[cslot * 3 + 2] = T(2) * aw * t2;
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           acc[3 + i] += T(2) * aw * u[i] * (u[i] * s - hr[i]);
@@ -258,6 +284,35 @@ static hipError_t launch_p(const PointLaunch& L, hipStream_t st) {
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
   if (L.rows == 0) return hipSuccess;
   return L.grad_grad_q ? launch_p<true>(L, st) : launch_p<false>(L, st);
+}
+
+// ------------------------------------------------------------------ dq from contributions
+// dq[q] = sum over k of contrib[q*K + k] in slot order (fixed association).
+template <typename T>
+__global__ void contrib_reduce_kernel(const T* __restrict__ contrib, int64_t rows, int K,
+                                      T* __restrict__ dq) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows * 3) return;
+  const int64_t q = i / 3;
+  const int c = static_cast<int>(i % 3);
+  const T* src = contrib + q * K * 3 + c;
+  T s = T(0);
+  for (int k = 0; k < K; ++k) s += src[k * 3];
+  dq[i] = s;
+}
+
+hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, void* dq, int32_t dtype,
+                                 hipStream_t st) {
+  if (rows == 0) return hipSuccess;
+  const int64_t n = rows * 3;
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+  if (dtype == AKNN_F32)
+    hipLaunchKernelGGL(contrib_reduce_kernel<float>, grid, block, 0, st, static_cast<const float*>(contrib),
+                       rows, static_cast<int>(K), static_cast<float*>(dq));
+  else
+    hipLaunchKernelGGL(contrib_reduce_kernel<double>, grid, block, 0, st, static_cast<const double*>(contrib),
+                       rows, static_cast<int>(K), static_cast<double*>(dq));
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ CSR transpose

@@ -347,8 +347,12 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     unsigned long long* __restrict__ stats) {
   const int lane = threadIdx.x;
   int64_t b, q0;
+  // XCD-aware order of the (spatially sorted) query tiles: each XCD's L2 then holds
+  // the point tiles of one region
+  const int64_t nblk = static_cast<int64_t>(gridDim.x) * gridDim.y;
+  const int64_t lin = xcd_block(static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x, nblk);
   if (qtiles) {
-    const int64_t t = blockIdx.x;
+    const int64_t t = lin;
     int64_t lo = 0, hi = cl.B - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
@@ -357,8 +361,8 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     b = lo;
     q0 = cl.q_begin(b) + (t - qtiles[b]) * 64;
   } else {
-    b = blockIdx.y;
-    q0 = cl.q_begin(b) + static_cast<int64_t>(blockIdx.x) * 64;
+    b = lin / gridDim.x;
+    q0 = cl.q_begin(b) + (lin % gridDim.x) * 64;
   }
   const int64_t qend = cl.q_end(b);
   const int64_t pos = q0 + lane;

@@ -323,11 +323,13 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     dims = make_dims(B, Q, P, F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
+    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     with torch.cuda.device(dev):
         _lib.check(lib.aknn_backward(ctypes.byref(dims), mask, grad_f_out.data_ptr(), q.data_ptr(),
                                      p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
                                      k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
-                                     P_(outs["dq"]),
+                                     P_(ws), ws_bytes, P_(outs["dq"]),
                                      P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
                                      P_(outs["dR"]), _stream(dev)))
     return [outs[n] for n in names]
@@ -364,12 +366,14 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
     dims = make_dims(B, Q, P, F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
+    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     with torch.cuda.device(dev):
         _lib.check(lib.aknn_backward2(ctypes.byref(dims), mask, grad_grad_q.data_ptr(),
                                       grad_f_out.data_ptr(), q.data_ptr(), p.data_ptr(),
                                       f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
                                       k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
-                                      P_(outs["dgf"]),
+                                      P_(ws), ws_bytes, P_(outs["dgf"]),
                                       P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]),
                                       P_(outs["dsigma"]), P_(outs["dR"]), _stream(dev)))
     return [outs[n] for n in names]

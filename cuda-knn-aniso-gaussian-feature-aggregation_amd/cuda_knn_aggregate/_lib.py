@@ -47,6 +47,7 @@ EXPORTS = (
     "aknn_aggregate",
     "aknn_csr_workspace_size",
     "aknn_build_csr",
+    "aknn_backward_workspace_size",
     "aknn_backward",
     "aknn_backward2",
 )
@@ -105,8 +106,11 @@ _SIGS = {
     "aknn_aggregate": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P] * 3 + [_P]),
     "aknn_csr_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_build_csr": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, ctypes.c_size_t, _P]),
-    "aknn_backward": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 9 + [_CACHE] + [_P] * 5 + [_P]),
-    "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE] + [_P] * 6 + [_P]),
+    "aknn_backward_workspace_size": (ctypes.c_size_t, [_DIMS, ctypes.c_uint32]),
+    "aknn_backward": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 9 + [_CACHE, _P, ctypes.c_size_t]
+                      + [_P] * 5 + [_P]),
+    "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE, _P, ctypes.c_size_t]
+                       + [_P] * 6 + [_P]),
 }
 
 

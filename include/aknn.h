@@ -108,8 +108,8 @@ typedef struct aknn_tune {
  * Sizes: query_order int32[query rows]; point_order int32[point rows];
  * point_records dtype[point rows * 16]. */
 typedef struct aknn_cache {
-  int32_t* query_order;
-  int32_t* point_order;
+  int32_t* query_order;  /* query rows in spatial order */
+  int32_t* point_order;  /* point rows in spatial order */
   void* point_records;
 } aknn_cache;
 
@@ -161,6 +161,11 @@ AKNN_API size_t aknn_csr_workspace_size(const aknn_dims* dims);
 AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t* csr_offsets,
                    int32_t* csr_entries, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Device workspace of aknn_backward / aknn_backward2 for a `needs` mask (0 when none is
+ * needed): with dq and point gradients both requested, the point kernels hand each
+ * (query, slot) entry's dq share to a per-query sum instead of a second gather. */
+AKNN_API size_t aknn_backward_workspace_size(const aknn_dims* dims, uint32_t needs);
+
 /* First-order backward.  Replaces knn_aggregate_aniso_backward
  * (csrc/knn_aggregate.cpp:101-132).  Gaussian weights are recomputed from idxs;
  * grad_w_out is not an input (the reference ignores it, csrc/backward.cu:93-94).
@@ -169,8 +174,8 @@ AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t*
 AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
-                  const aknn_cache* cache, void* dq, void* dp, void* df, void* dsigma, void* dR,
-                  void* stream);
+                  const aknn_cache* cache, void* workspace, size_t workspace_bytes, void* dq,
+                  void* dp, void* df, void* dsigma, void* dR, void* stream);
 
 /* Second-order backward of <grad_grad_q, dq>.  Replaces knn_aggregate_aniso_backward_2nd
  * (csrc/knn_aggregate.cpp:135-172); like the reference only grad_grad_q is an input
@@ -179,8 +184,9 @@ AKNN_API int aknn_backward2(const aknn_dims* dims, uint32_t needs, const void* g
                    const void* grad_f_out, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs,
                    const int32_t* csr_offsets, const int32_t* csr_entries,
-                   const aknn_cache* cache, void* d_grad_f_out, void* dq, void* dp, void* df,
-                   void* dsigma, void* dR, void* stream);
+                   const aknn_cache* cache, void* workspace, size_t workspace_bytes,
+                   void* d_grad_f_out, void* dq, void* dp, void* df, void* dsigma, void* dR,
+                   void* stream);
 
 #ifdef __cplusplus
 }

@@ -69,13 +69,18 @@ def test_backward_rejects_missing_outputs_and_csr():
     d = _dims()
     fake = ctypes.c_void_p(0x1000)
     rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DQ, *([fake] * 7), None, None, None,
-                           None, None, None, None, None, None)
+                           None, 0, None, None, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "dq requested" in lib.aknn_last_error().decode()
     rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DP, *([fake] * 7), None, None, None,
-                           None, fake, None, None, None, None)
+                           None, 0, None, fake, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "CSR" in lib.aknn_last_error().decode()
+    rc = lib.aknn_backward(ctypes.byref(d), _lib.NEED_DP | _lib.NEED_DQ, *([fake] * 7), fake, fake, None,
+                           None, 0, fake, fake, None, None, None, None)
+    assert rc == _lib.AKNN_E_WORKSPACE
+    assert lib.aknn_backward_workspace_size(ctypes.byref(d), _lib.NEED_DP | _lib.NEED_DQ) >= 2 * 10 * 3 * 3 * 4
+    assert lib.aknn_backward_workspace_size(ctypes.byref(d), _lib.NEED_DQ) == 0
     rc = lib.aknn_backward2(ctypes.byref(d), _lib.NEED_DQ, None, *([fake] * 7), None, None, None,
-                            None, fake, None, None, None, None, None)
+                            None, 0, None, fake, None, None, None, None, None)
     assert rc == _lib.AKNN_E_INVALID and "grad_grad_q" in lib.aknn_last_error().decode()
 
 
