@@ -93,6 +93,12 @@ const uint32_t* pruned_point_order(const PrunedLaunch& L);
 hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, void* dq, int32_t dtype,
                                  hipStream_t st);
 
+// Stable LSD radix sort (aknn_sort.hip): vals_in NULL sorts the positions 0..n-1.
+size_t radix_sort_workspace(int64_t n, size_t key_bytes);
+template <typename K>
+hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                            int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st);
+
 // CSR build
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);
 hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,

@@ -15,9 +15,6 @@
 // Second order with h = grad_grad_q[q], s = h.v, t = 2 s v - M h (csrc/backward.cu:196-326):
 //   dp += -2 a w t;  dsigma_i += 2 a w u_i (u_i s - h.R_i);
 //   dR_ij += -2 a w s_i (h_j u_i + (h.R_i) d_j - 2 s u_i d_j);  df += -2 w s g
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-
 #include "aknn_common.h"
 #include "aknn_internal.h"
 
@@ -349,35 +346,28 @@ static int key_bits(int64_t rows) {
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes) {
-  size_t sort_bytes = 0;
-  rocprim::counting_iterator<uint32_t> vin(0);
-  const hipError_t e = rocprim::radix_sort_pairs(
-      nullptr, sort_bytes, static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-      vin, static_cast<uint32_t*>(nullptr), static_cast<size_t>(entries), 0, key_bits(point_rows),
-      nullptr);
-  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(sort_bytes);
-  return e;
+  (void)point_rows;
+  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(radix_sort_workspace(entries, 4));
+  return hipSuccess;
 }
 
 hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
                             int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
                             size_t ws_bytes, hipStream_t st) {
+  (void)ws_bytes;
   const int64_t entries = query_rows * K;
   char* base = static_cast<char*>(ws);
   uint32_t* keys_in = reinterpret_cast<uint32_t*>(base);
   uint32_t* keys_out = reinterpret_cast<uint32_t*>(base + align256(entries * 4));
   void* tmp = base + 2 * align256(entries * 4);
-  size_t tmp_bytes = ws_bytes - 2 * align256(entries * 4);
   if (entries > 0) {
     const int bs = 256;
     hipLaunchKernelGGL(csr_keys_kernel, dim3(static_cast<unsigned>((entries + bs - 1) / bs)), dim3(bs),
                        0, st, idxs, cl, K, entries, keys_in);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    rocprim::counting_iterator<uint32_t> vin(0);
-    err = rocprim::radix_sort_pairs(tmp, tmp_bytes, static_cast<const uint32_t*>(keys_in), keys_out,
-                                    vin, reinterpret_cast<uint32_t*>(csr_ent),
-                                    static_cast<size_t>(entries), 0, key_bits(point_rows), st);
+    err = radix_sort_pairs<uint32_t>(keys_in, keys_out, nullptr, reinterpret_cast<uint32_t*>(csr_ent),
+                                     entries, 0, key_bits(point_rows), tmp, st);
     if (err != hipSuccess) return err;
   }
   const int bs = 256;

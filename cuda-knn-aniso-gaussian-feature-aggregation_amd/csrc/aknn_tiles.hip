@@ -21,9 +21,6 @@
 // lmin comes from the closed-form 3x3 symmetric eigenvalues in FP64 minus a margin, the
 // result is scaled by (1 - 1e-6) to absorb the FP32 rounding of the tile test itself.
 // Points with any negative scale, or non-finite data, mark their tile "always scan".
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-
 #include "aknn_common.h"
 #include "aknn_internal.h"
 
@@ -67,13 +64,15 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   return v;
 }
 
+// 24-bit Morton code (8 bits per axis) in the cloud's box.
+constexpr int kCodeBits = 24;
 template <typename T>
 __device__ __forceinline__ uint32_t morton(T x, T y, T z, const float* box) {
   auto q = [](T v, float lo, float hi) -> uint32_t {
     const float ext = hi - lo;
     float t = ext > 0.f ? (static_cast<float>(v) - lo) / ext : 0.f;
     t = t != t ? 0.f : fminf(fmaxf(t, 0.f), 1.f);  // NaN -> 0; clamp queries outside the box
-    return static_cast<uint32_t>(t * 1023.f);
+    return static_cast<uint32_t>(t * 255.f);
   };
   return (spread10(q(x, box[0], box[3])) << 2) | (spread10(q(y, box[1], box[4])) << 1) |
          spread10(q(z, box[2], box[5]));
@@ -148,7 +147,7 @@ __global__ void box_final_kernel(const unsigned* __restrict__ ob, int64_t B, flo
   box[i] = v;
 }
 
-// keys[i] = (cloud << 32) | morton(row i); rows are query rows (QUERY) or point rows.
+// keys[i] = (cloud << 24) | morton(row i); rows are query rows (QUERY) or point rows.
 template <typename T, bool QUERY>
 __global__ void codes_kernel(const T* __restrict__ xyz, Clouds cl, int64_t rows,
                              const float* __restrict__ box, uint64_t* __restrict__ keys) {
@@ -156,7 +155,7 @@ __global__ void codes_kernel(const T* __restrict__ xyz, Clouds cl, int64_t rows,
   if (i >= rows) return;
   const int64_t b = QUERY ? cl.cloud_of_query(i) : cl.cloud_of_point(i);
   const uint32_t c = morton<T>(xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2], box + b * 8);
-  keys[i] = (static_cast<uint64_t>(b) << 32) | c;
+  keys[i] = (static_cast<uint64_t>(b) << kCodeBits) | c;
 }
 
 // ------------------------------------------------------------------ point bound
@@ -533,20 +532,9 @@ static hipError_t launch_pruned_topk(const PrunedLaunch& L, const PrunedState& S
 }
 
 static int key_bits64(int64_t B) {
-  int bits = 30;  // 30-bit Morton codes below the cloud id
-  if (B > 1) {
-    bits = 32;
-    while ((int64_t(1) << (bits - 32)) < B) ++bits;
-  }
+  int bits = kCodeBits;  // Morton code below the cloud id
+  while ((int64_t(1) << (bits - kCodeBits)) < B) ++bits;
   return bits;
-}
-
-static hipError_t sort_bytes(int64_t n, int bits, size_t* out) {
-  rocprim::counting_iterator<uint32_t> vin(0);
-  return rocprim::radix_sort_pairs(nullptr, *out, static_cast<const uint64_t*>(nullptr),
-                                   static_cast<uint64_t*>(nullptr), vin,
-                                   static_cast<uint32_t*>(nullptr), static_cast<size_t>(n), 0, bits,
-                                   nullptr);
 }
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -588,13 +576,9 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
 }
 
 static hipError_t sort_temp(const PrunedLaunch& L, size_t* bytes) {
-  const int bits = key_bits64(L.cl.B);
-  size_t sp = 0, sq = 0;
-  hipError_t e = sort_bytes(L.prows, bits, &sp);
-  if (e != hipSuccess) return e;
-  e = sort_bytes(L.qrows, bits, &sq);
-  *bytes = sp > sq ? sp : sq;
-  return e;
+  const int64_t n = L.prows > L.qrows ? L.prows : L.qrows;
+  *bytes = radix_sort_workspace(n, 8);
+  return hipSuccess;
 }
 
 hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes) {
@@ -656,14 +640,9 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
                      st, q, L.cl, L.qrows, S.box, S.qkeys);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  rocprim::counting_iterator<uint32_t> vin(0);
-  size_t tb = S.tmp_bytes;
-  e = rocprim::radix_sort_pairs(S.tmp, tb, static_cast<const uint64_t*>(S.pkeys), S.pkeys_s, vin, S.pperm,
-                                static_cast<size_t>(L.prows), 0, bits, st);
+  e = radix_sort_pairs<uint64_t>(S.pkeys, S.pkeys_s, nullptr, S.pperm, L.prows, 0, bits, S.tmp, st);
   if (e != hipSuccess) return e;
-  tb = S.tmp_bytes;
-  e = rocprim::radix_sort_pairs(S.tmp, tb, static_cast<const uint64_t*>(S.qkeys), S.qkeys_s, vin, S.qperm,
-                                static_cast<size_t>(L.qrows), 0, bits, st);
+  e = radix_sort_pairs<uint64_t>(S.qkeys, S.qkeys_s, nullptr, S.qperm, L.qrows, 0, bits, S.tmp, st);
   if (e != hipSuccess) return e;
   if (L.cl.p_off) {
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.p_off, L.cl.B, int64_t(kTile), S.ptp);

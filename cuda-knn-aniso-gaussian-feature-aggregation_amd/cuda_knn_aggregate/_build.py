@@ -27,6 +27,7 @@ SOURCES = [
     "aknn_agg.hip",
     "aknn_points.hip",
     "aknn_tiles.hip",
+    "aknn_sort.hip",
 ]
 ARCH = "gfx950"
 

@@ -1,0 +1,232 @@
+"""Ragged batches: many point clouds of uneven sizes in one call (offset-packed).
+
+All rows of all clouds are concatenated (``q [sum Q_i, 3]``, ``p [sum P_i, 3]``,
+``f [sum P_i, F]``, ``sigma [sum P_i, 3]``, ``R [sum P_i, 3, 3]``) and described by int64
+offsets (``q_offsets[b] .. q_offsets[b+1]`` are cloud b's query rows).  Every kernel
+covers the whole batch in one launch (no per-cloud loop); ``idxs`` are point indices
+within their own cloud, as in the dense call.  Semantics per cloud are those of
+:func:`cuda_knn_aggregate.knn_aggregate_aniso`.
+
+The :class:`RaggedLayout` holds the host and device copies of the offsets; build it once
+(outside any graph capture) and reuse it: the calls then synchronise nothing and
+allocate only through torch, so a whole fwd + bwd + 2nd-order step can be captured.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _C, _lib
+
+__all__ = ["RaggedLayout", "knn_aggregate_aniso_ragged", "KNNAnisotropicAggregateRagged",
+           "KNNAnisotropicAggregateRaggedBackward"]
+
+
+class RaggedLayout:
+    """Offsets of a ragged batch (host and device copies)."""
+
+    def __init__(self, q_offsets, p_offsets, device):
+        qh = torch.as_tensor(q_offsets, dtype=torch.int64).detach().cpu().contiguous()
+        ph = torch.as_tensor(p_offsets, dtype=torch.int64).detach().cpu().contiguous()
+        if qh.dim() != 1 or ph.dim() != 1 or qh.numel() != ph.numel() or qh.numel() < 2:
+            raise RuntimeError("q_offsets and p_offsets must be 1-D with the same length B+1 >= 2")
+        if int(qh[0]) != 0 or int(ph[0]) != 0:
+            raise RuntimeError("offsets must start at 0")
+        if bool((qh[1:] < qh[:-1]).any()) or bool((ph[1:] < ph[:-1]).any()):
+            raise RuntimeError("offsets must be non-decreasing")
+        self.q_off_host, self.p_off_host = qh, ph
+        self.device = torch.device(device)
+        self.q_off = qh.to(self.device)
+        self.p_off = ph.to(self.device)
+        self.B = qh.numel() - 1
+        self.Q = int(qh[-1])
+        self.P = int(ph[-1])
+        self.q_sizes = (qh[1:] - qh[:-1]).tolist()
+        self.p_sizes = (ph[1:] - ph[:-1]).tolist()
+        self.min_points = min(self.p_sizes)
+
+    def dims(self, F, K, dtype, key="aniso"):
+        return _C.make_dims(self.B, self.Q, self.P, F, K, dtype, key,
+                            offsets=(self.q_off, self.p_off, self.q_off_host, self.p_off_host))
+
+    def new_cache(self, dtype):
+        return _C.new_cache(1, self.Q, self.P, dtype, self.device)
+
+
+def _check(layout, q, p, f, sigma, R):
+    for t, n in zip((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R")):
+        _C._check_input(t, n)
+    dt = _C._check_float((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R"))
+    if tuple(q.shape) != (layout.Q, 3):
+        raise RuntimeError(f"q must have shape [{layout.Q}, 3] (got {tuple(q.shape)})")
+    P = layout.P
+    if tuple(p.shape) != (P, 3) or tuple(sigma.shape) != (P, 3) or tuple(R.shape) != (P, 3, 3):
+        raise RuntimeError("p, sigma, R must have shapes [P,3], [P,3], [P,3,3] with P = sum of point counts")
+    if f.dim() != 2 or f.shape[0] != P or f.shape[1] < 1:
+        raise RuntimeError(f"f must have shape [{P}, F]")
+    if q.device != layout.device:
+        raise RuntimeError(f"tensors must be on the layout's device {layout.device}")
+    return dt, f.shape[1]
+
+
+def _check_k(layout, K):
+    K = int(K)
+    if K < 1 or K > 64:
+        raise RuntimeError(f"K must be in [1, 64] (got {K})")
+    if K > layout.min_points:
+        raise RuntimeError(f"K ({K}) must not exceed the smallest cloud's point count ({layout.min_points})")
+    return K
+
+
+def forward(layout, q, p, f, sigma, R, K, *, key="aniso", tune=None, cache=None):
+    """-> ``[f_out [Q,F], w_out [Q,K], dists [Q,K], idxs [Q,K] (int64, within-cloud)]``."""
+    dt, F = _check(layout, q, p, f, sigma, R)
+    K = _check_k(layout, K)
+    dev = q.device
+    f_out = torch.empty(layout.Q, F, dtype=dt, device=dev)
+    w_out = torch.empty(layout.Q, K, dtype=dt, device=dev)
+    dists = torch.empty(layout.Q, K, dtype=dt, device=dev)
+    idxs = torch.empty(layout.Q, K, dtype=torch.int64, device=dev)
+    if layout.Q == 0:
+        return [f_out, w_out, dists, idxs]
+    lib = _lib.load()
+    dims = layout.dims(F, K, dt, key)
+    ws_bytes = lib.aknn_forward_workspace_size(ctypes.byref(dims))
+    if ws_bytes == 0:
+        _lib.check(_lib.AKNN_E_LAUNCH)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    tn = _C._tune(tune)
+    cs = _C._cache_struct(cache)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_forward(ctypes.byref(dims), _C._tp(tn), q.data_ptr(), p.data_ptr(), f.data_ptr(),
+                                    sigma.data_ptr(), R.data_ptr(), f_out.data_ptr(), w_out.data_ptr(),
+                                    dists.data_ptr(), idxs.data_ptr(), ws.data_ptr(), ws_bytes, _C._cp(cs),
+                                    _C._stream(dev)))
+    return [f_out, w_out, dists, idxs]
+
+
+def build_csr(layout, idxs, K):
+    dev = idxs.device
+    off = torch.empty(layout.P + 1, dtype=torch.int32, device=dev)
+    ent = torch.empty(max(layout.Q * K, 1), dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    dims = layout.dims(1, K, torch.float32)
+    ws_bytes = lib.aknn_csr_workspace_size(ctypes.byref(dims))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_build_csr(ctypes.byref(dims), idxs.data_ptr(), off.data_ptr(), ent.data_ptr(),
+                                      ws.data_ptr(), ws_bytes, _C._stream(dev)))
+    return off, ent
+
+
+def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, needs, cache):
+    dt, F = _check(layout, q, p, f, sigma, R)
+    K = int(idxs.shape[1])
+    names = ("dgf", "dq", "dp", "df", "dsigma", "dR") if second else ("dq", "dp", "df", "dsigma", "dR")
+    mask = _C._needs_mask(needs, names)
+    dev = q.device
+    Q, P = layout.Q, layout.P
+    shapes = {"dgf": (Q, F), "dq": (Q, 3), "dp": (P, 3), "df": (P, F), "dsigma": (P, 3), "dR": (P, 3, 3)}
+    outs = {n: (torch.empty(shapes[n], dtype=dt, device=dev) if mask & _C._NEED_BITS[n] else None) for n in names}
+    if mask & _lib.NEED_POINTS and csr is None:
+        csr = build_csr(layout, idxs, K)
+    lib = _lib.load()
+    dims = layout.dims(F, K, dt, key)
+    off, ent = csr if csr is not None else (None, None)
+    P_ = _lib.ptr
+    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
+    cs = _C._cp(_C._cache_struct(cache))
+    with torch.cuda.device(dev):
+        if second:
+            rc = lib.aknn_backward2(ctypes.byref(dims), mask, gg.data_ptr(), g.data_ptr(), q.data_ptr(),
+                                    p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(), idxs.data_ptr(),
+                                    P_(off), P_(ent), cs, P_(ws), ws_bytes, P_(outs["dgf"]), P_(outs["dq"]),
+                                    P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]), P_(outs["dR"]),
+                                    _C._stream(dev))
+        else:
+            rc = lib.aknn_backward(ctypes.byref(dims), mask, g.data_ptr(), q.data_ptr(), p.data_ptr(),
+                                   f.data_ptr(), sigma.data_ptr(), R.data_ptr(), idxs.data_ptr(), P_(off),
+                                   P_(ent), cs, P_(ws), ws_bytes, P_(outs["dq"]), P_(outs["dp"]),
+                                   P_(outs["df"]), P_(outs["dsigma"]), P_(outs["dR"]), _C._stream(dev))
+        _lib.check(rc)
+    return [outs[n] for n in names]
+
+
+def backward(layout, g, q, p, f, sigma, R, idxs, *, key="aniso", csr=None, needs=None, cache=None):
+    """First-order gradients -> ``[dq, dp, df, dsigma, dR]``."""
+    return _backward_impl(layout, False, None, g.contiguous(), q, p, f, sigma, R, idxs, key, csr, needs, cache)
+
+
+def backward2(layout, gg, g, q, p, f, sigma, R, idxs, *, key="aniso", csr=None, needs=None, cache=None):
+    """Second-order gradients of ``<gg, dq>`` -> ``[d grad_f_out, dq, dp, df, dsigma, dR]``."""
+    return _backward_impl(layout, True, gg.contiguous(), g.contiguous(), q, p, f, sigma, R, idxs, key, csr,
+                          needs, cache)
+
+
+class KNNAnisotropicAggregateRagged(torch.autograd.Function):
+    """Ragged counterpart of :class:`cuda_knn_aggregate.KNNAnisotropicAggregate`."""
+
+    @staticmethod
+    def forward(ctx, q, p, f, sigma, R, layout, K, key="aniso", tune=None):
+        cache = layout.new_cache(q.dtype) if any(ctx.needs_input_grad[:5]) else None
+        f_out, w_out, dists, idxs = forward(layout, q, p, f, sigma, R, K, key=key, tune=tune, cache=cache)
+        ctx.save_for_backward(q, p, f, sigma, R, idxs, *(cache or ()))
+        ctx.layout, ctx.key, ctx.has_cache = layout, key, cache is not None
+        ctx.mark_non_differentiable(w_out, dists, idxs)
+        ctx.set_materialize_grads(False)
+        return f_out, w_out, dists, idxs
+
+    @staticmethod
+    def backward(ctx, grad_f_out, grad_w_out, grad_dists, grad_idxs):
+        q, p, f, sigma, R, idxs = ctx.saved_tensors[:6]
+        cache = tuple(ctx.saved_tensors[6:9]) if ctx.has_cache else None
+        if grad_f_out is None:
+            return (None,) * 9
+        needs = tuple(ctx.needs_input_grad[:5])
+        grads = KNNAnisotropicAggregateRaggedBackward.apply(grad_f_out.contiguous(), q, p, f, sigma, R, idxs,
+                                                            ctx.layout, ctx.key, needs, cache)
+        return tuple(grads) + (None,) * 4
+
+
+class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
+    """First order as a differentiable op; its backward is the Eikonal second order
+    (only the gradient arriving on dq is honoured, as in the dense Function)."""
+
+    @staticmethod
+    def forward(ctx, grad_f_out, q, p, f, sigma, R, idxs, layout, key, needs, cache):
+        csr = build_csr(layout, idxs, int(idxs.shape[1])) if any(needs[1:]) else None
+        out = backward(layout, grad_f_out, q, p, f, sigma, R, idxs, key=key, csr=csr, needs=needs, cache=cache)
+        saved = [grad_f_out, q, p, f, sigma, R, idxs] + (list(csr) if csr else []) + (list(cache) if cache else [])
+        ctx.save_for_backward(*saved)
+        ctx.layout, ctx.key, ctx.has_csr, ctx.has_cache = layout, key, csr is not None, cache is not None
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, gg_q, gg_p, gg_f, gg_sigma, gg_R):
+        saved = list(ctx.saved_tensors)
+        grad_f_out, q, p, f, sigma, R, idxs = saved[:7]
+        rest = saved[7:]
+        csr = (rest.pop(0), rest.pop(0)) if ctx.has_csr else None
+        cache = tuple(rest[:3]) if ctx.has_cache else None
+        ni = ctx.needs_input_grad
+        needs = (ni[0], ni[1], ni[2], ni[3], ni[4], ni[5])
+        if gg_q is None or not any(needs):
+            return (None,) * 11
+        if csr is None and any(needs[2:]):
+            csr = build_csr(ctx.layout, idxs, int(idxs.shape[1]))
+        out = backward2(ctx.layout, gg_q, grad_f_out, q, p, f, sigma, R, idxs, key=ctx.key, csr=csr,
+                        needs=needs, cache=cache)
+        return tuple(out) + (None,) * 5
+
+
+def knn_aggregate_aniso_ragged(q, p, f, sigma, R, layout: RaggedLayout, K: int, *, key: str = "aniso",
+                               tune=None):
+    """k-NN anisotropic aggregation over a ragged batch of clouds (one launch per kernel).
+
+    Returns ``f_out [sum Q_i, F]``, ``w_out``, ``dists``, ``idxs [sum Q_i, K]`` (indices
+    within each query's own cloud); differentiable in q, p, f, sigma, R with the same first-
+    and second-order semantics as the dense call.
+    """
+    return KNNAnisotropicAggregateRagged.apply(q, p, f, sigma, R, layout, K, key, tune)

@@ -322,3 +322,20 @@ def test_cache_does_not_change_results(cuda, search):
     s2 = _C.knn_aggregate_aniso_backward_2nd(h, b.g, w_out, *args, w_out, idxs, cache=cache)
     for x, y in zip(s1, s2):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("P,Q,K", [(300, 700, 8), (65536, 20000, 16), (5000, 300000, 16)])
+def test_csr_transpose(cuda, P, Q, K):
+    """The point-side transpose of idxs: every entry once, each row's entries ascending
+    (the fixed reduction order), offsets = exclusive scan of the per-point counts."""
+    g = torch.Generator().manual_seed(P)
+    idxs = torch.randint(P, (1, Q, K), generator=g)
+    idxs[0, : Q // 3, 0] = 7  # one heavy row
+    off, ent = _C.build_csr(idxs.to(cuda), P)
+    off, ent = off.cpu().long(), ent.cpu().long()
+    counts = torch.bincount(idxs.reshape(-1), minlength=P)
+    assert torch.equal(off[1:] - off[:-1], counts) and off[0] == 0
+    key = idxs.reshape(-1)[ent]                      # point row of each placed entry
+    assert torch.equal(key, torch.sort(idxs.reshape(-1), stable=True).values)
+    ref = torch.sort(idxs.reshape(-1), stable=True).indices
+    assert torch.equal(ent, ref)
