@@ -33,6 +33,8 @@ WORKLOADS = {
     "clustered": "clustered B=1 Q=1048576 P=65536 K=16 F=64",
     "bench_small": "uniform B=2 Q=12288 P=4096 K=16 F=64",
     "readme": "uniform B=2 Q=1000 P=500 K=12 F=32",
+    "ragged": "ragged 64 clouds P_i~logU[1024,32768] Q_i~logU[4096,131072] K=16 F=32",
+    "serving": "serving 16 resident clouds P=65536 F=64 K=16; Poisson requests Q~logU[256,65536]; fwd + dq",
 }
 
 
@@ -98,6 +100,125 @@ def cpu_baseline(name, budget_s=15.0):
                       f"float32, best of 2 runs ({t:.2f} s)"}
 
 
+def ragged_bench(args):
+    """Ragged batch: one batched fwd+1st+2nd-order step over all clouds (one launch per
+    kernel) vs a per-cloud loop of the plain single-cloud call on the same GPU."""
+    import torch
+
+    from cuda_knn_aggregate import _lib, knn_aggregate_aniso, workloads
+    from cuda_knn_aggregate.ragged import RaggedLayout, knn_aggregate_aniso_ragged
+
+    dev = torch.device("cuda:0")
+    r = workloads.ragged_batch(seed=0)
+    layout = RaggedLayout(r.q_off, r.p_off, dev)
+    d = r.to(dev)
+    params = [t.clone().requires_grad_(True) for t in (d.q, d.p, d.f, d.sigma, d.R)]
+    v = d.v
+
+    from cuda_knn_aggregate import autotune
+
+    tune = autotune.tuned(d.q, d.p, d.sigma, d.R, r.K, layout=layout)
+    print(f"[bench] autotuned search: {tune}", flush=True)
+
+    def batched():
+        f_out, *_ = knn_aggregate_aniso_ragged(*params, layout, r.K, tune=tune)
+        dq = torch.autograd.grad(f_out, params[0], v.expand_as(f_out), create_graph=True)[0]
+        L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+        return torch.autograd.grad(L, params)
+
+    clouds = []
+    for b in range(r.B):
+        c = r.cloud(b).to(dev)
+        clouds.append([t.contiguous().clone().requires_grad_(True) for t in (c.q, c.p, c.f, c.sigma, c.R)])
+
+    def per_cloud():
+        for cp in clouds:
+            f_out, *_ = knn_aggregate_aniso(*cp, r.K)
+            dq = torch.autograd.grad(f_out, cp[0], v.expand_as(f_out), create_graph=True)[0]
+            L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+            torch.autograd.grad(L, cp)
+
+    def timed(fn, steps, warm):
+        for _ in range(warm):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    print(f"[bench] ragged: {r.B} clouds, {layout.Q} queries, {layout.P} points", flush=True)
+    with _lib.trace() as tr:
+        tb = timed(batched, args.steps, args.warmup)
+    tl = timed(per_cloud, max(1, args.steps // 4), 1)
+    kernels = {k: {"launches": n, "avg_ms": tot / n} for k, (n, tot) in tr.summary().items()}
+    out = {
+        "metric": METRIC, "value": layout.Q / tb, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tb * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["ragged"], "B": r.B, "Q_total": layout.Q, "P_total": layout.P,
+                   "K": r.K, "F": int(r.f.shape[1]), "step": "fwd + 1st-order (create_graph) + 2nd-order Eikonal"},
+        "per_cloud_loop": {"value": layout.Q / tl, "ms_per_step": tl * 1e3},
+        "batched_over_loop": tl / tb,
+        "tune": tune,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serving_bench(args):
+    """Resident clouds, Poisson arrivals at several rates, continuous batching; p50/p99
+    latency (arrival to results on the host) and served queries/s per rate."""
+    import math
+
+    import torch
+
+    from cuda_knn_aggregate import workloads
+    from cuda_knn_aggregate.serving import Request, Server, latency_stats
+
+    dev = torch.device("cuda:0")
+    clouds = []
+    for c in range(16):
+        b = workloads.clustered_batch(Q=1, P=65536, K=16, F=64, seed=200 + c)
+        clouds.append((b.p[0], b.f[0], b.sigma[0], b.R[0]))
+    v = torch.randn(64, generator=torch.Generator().manual_seed(9))
+    server = Server(clouds, 16, v, device=dev, streams=4, hbm_budget=8 << 30, max_batch_queries=1 << 17)
+
+    def requests(n, rate, seed):
+        g = torch.Generator().manual_seed(seed)
+        out, t = [], 0.0
+        lo, hi = math.log(256), math.log(65536)
+        for i in range(n):
+            c = int(torch.randint(16, (1,), generator=g))
+            Q = int(round(math.exp(lo + (hi - lo) * float(torch.rand(1, generator=g)))))
+            p = clouds[c][0]
+            q = p[torch.randint(p.shape[0], (Q,), generator=g)] + 0.01 * torch.randn(Q, 3, generator=g)
+            t += float(torch.empty(1).exponential_(rate, generator=g))
+            out.append(Request(c, q.pin_memory(), arrival=t, rid=i))
+        return out
+
+    server.replay(requests(40, 200.0, 1))       # warm-up (code objects, allocator, pinned pool)
+    rows = []
+    for rate in (250.0, 500.0, 1000.0, 2000.0):
+        n = int(min(2000, max(200, rate * 2.0)))
+        reqs = requests(n, rate, int(rate))
+        b0 = server.batches
+        server.replay(reqs)
+        st = latency_stats(reqs)
+        st.update(rate_req_per_s=rate, batches=server.batches - b0,
+                  offered_queries_per_s=rate * sum(r.q.shape[0] for r in reqs) / len(reqs))
+        print(f"[bench] serving rate {rate}: {st}", flush=True)
+        rows.append(st)
+    best = max(rows, key=lambda r: r["queries_per_s"] if r["p99_ms"] < 5.0 else 0.0)
+    out = {"metric": METRIC, "value": best["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
+           "steps": sum(r["requests"] for r in rows), "warmup": 40, "ms_per_step": best["p50_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": {"workload": WORKLOADS["serving"], "streams": 4},
+           "served_p50_ms": best["p50_ms"], "served_p99_ms": best["p99_ms"], "rates": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +232,13 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by "
               "query across GPUs, which this build does not launch.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ragged":
+        ragged_bench(args)
+        return
+    if args.workload == "serving":
+        serving_bench(args)
+        return
 
     import torch
 

@@ -215,12 +215,12 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   return L;
 }
 
-// Default search: the tiled search once a cloud holds enough points for pruning to
-// pay (measured crossover), the brute-force scan below that.
+// Default search: the tiled search once clouds hold enough points for pruning to pay
+// (measured crossover ~8K points per cloud), the brute-force scan below that.
 int pick_search(const aknn_dims* d, const aknn_tune* t) {
   if (t && (t->search == AKNN_SEARCH_BRUTE || t->search == AKNN_SEARCH_TILES)) return t->search;
-  const int64_t pmin = min_points(d);
-  return pmin >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
+  const int64_t mean_p = d->B > 0 ? point_rows(d) / d->B : 0;
+  return mean_p >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
 }
 
 // Neighbour search.  On return *qorder / *porder point at spatial orders of the query

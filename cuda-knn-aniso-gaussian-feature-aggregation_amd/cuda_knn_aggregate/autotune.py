@@ -1,0 +1,111 @@
+"""Autotuner for the neighbour search launch.
+
+Tunables (``aknn_tune``): the search algorithm (brute-force scan or tiled search), and
+for the brute scan the queries held per lane (1, 2, 4 — register lists per lane) and
+the number of waves splitting a query tile's points (1..8 — their lists merge through
+LDS).  Every candidate returns the same neighbours; only the time differs.  The tuner
+times each candidate on the caller's own inputs (hipEvents on the current stream,
+median of a few runs), keeps the fastest per shape class in memory and in a JSON file
+(``$AKNN_TUNE_CACHE`` or ``~/.cache/cuda_knn_aggregate/autotune.json``), and returns a
+``tune`` dict accepted by every forward entry point.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import threading
+
+import torch
+
+from . import _C
+
+CANDIDATES = (
+    [{"search": "tiles"}]
+    + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]
+)
+
+_lock = threading.Lock()
+_mem: dict = {}
+
+
+def cache_path() -> str:
+    return os.environ.get("AKNN_TUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "cuda_knn_aggregate", "autotune.json")
+
+
+def _load():
+    try:
+        with open(cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(d):
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(d, fh, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses persistence
+
+
+def shape_class(B, Q_per_cloud, P_per_cloud, K, dtype, key, ragged) -> str:
+    """Shapes within a factor of ~1.4 (half-octave buckets) share a tuning."""
+    def bucket(x):
+        return int(round(2 * math.log2(max(1.0, x))))
+    dev = torch.cuda.get_device_name() if torch.cuda.is_available() else "cpu"
+    return f"{dev}|{str(dtype).split('.')[-1]}|{key}|K{K}|B{bucket(B)}|Q{bucket(Q_per_cloud)}|P{bucket(P_per_cloud)}|r{int(ragged)}"
+
+
+def _time(fn, reps=3):
+    ts = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def tuned(q, p, sigma, R, K, *, key="aniso", layout=None, candidates=CANDIDATES, force=False):
+    """The fastest search launch for these inputs' shape class (timed on first use)."""
+    if layout is None:
+        B, Qc, Pc = q.shape[0], q.shape[1], p.shape[1]
+    else:
+        B, Qc, Pc = layout.B, layout.Q / max(1, layout.B), layout.P / max(1, layout.B)
+    cls = shape_class(B, Qc, Pc, K, q.dtype, key, layout is not None)
+    with _lock:
+        if not _mem:
+            _mem.update(_load())
+        if cls in _mem and not force:
+            return dict(_mem[cls]["tune"])
+    if layout is None:
+        def run(t):
+            _C.knn_aniso(q, p, sigma, R, K, key=key, tune=t)
+    else:
+        from . import ragged as rg
+        f1 = torch.empty(layout.P, 1, dtype=q.dtype, device=q.device)
+
+        def run(t):
+            rg.forward(layout, q, p, f1, sigma, R, K, key=key, tune=t)
+    results = []
+    for t in candidates:
+        run(t)  # warm-up (code object load, allocator)
+        results.append((_time(lambda: run(t)), t))
+    results.sort(key=lambda x: x[0])
+    best_ms, best = results[0]
+    with _lock:
+        _mem[cls] = {"tune": best, "ms": best_ms, "all": [(ms, t) for ms, t in results]}
+        disk = _load()
+        disk[cls] = _mem[cls]
+        _save(disk)
+    return dict(best)
