@@ -189,6 +189,8 @@ size_t rec_bytes(const aknn_dims* d) { return align256(static_cast<size_t>(point
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+constexpr int64_t kSplitBlocks = 16384;
+
 aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   aknn::PrunedLaunch L{};
   L.dtype = d->dtype;
@@ -212,6 +214,18 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
     L.nqtiles = d->B * ceil_div(d->Q, 64);
     L.max_points = d->P;
   }
+  // Few query tiles (small or scattered query sets) cannot fill the device and the
+  // slowest tile's walk sets the kernel time: split every tile's walk over up to 64
+  // blocks with at least 8 point tiles per block and cloud, and at most kSplitBlocks
+  // blocks in all (the workspace is sized for that largest split, ways_max).  The
+  // default follows ways^2 * query tiles ~ 65536, the measured optimum on 64K-point
+  // clouds from 4 to 4096 query tiles; a tuning value may pick anything up to ways_max.
+  const int64_t tiles_per_cloud = L.ntiles / std::max<int64_t>(1, d->B);
+  int w = 1;
+  while (w < 64 && L.nqtiles * (2 * w) <= kSplitBlocks && tiles_per_cloud >= 8 * (2 * w)) w *= 2;
+  L.ways_max = w;
+  L.ways = 1;
+  while (2 * L.ways <= L.ways_max && L.nqtiles * (4 * L.ways * L.ways) <= 65536) L.ways *= 2;
   return L;
 }
 
@@ -242,6 +256,11 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.ws = ws;
     PL.ws_bytes = ws_bytes;
     PL.stats = tune ? static_cast<unsigned long long*>(tune->stats) : nullptr;
+    if (tune && tune->point_split > 0) {
+      int w = 1;
+      while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;
+      PL.ways = w;
+    }
     hipError_t e;
     {
       Span sp("knn_tiles_prep", st);

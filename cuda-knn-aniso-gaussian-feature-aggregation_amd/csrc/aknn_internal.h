@@ -30,6 +30,8 @@ struct PrunedLaunch {
   int64_t nsuper;         // super-tiles (sum over clouds of ceil(tiles_b / 16))
   int64_t nqtiles;        // query tiles (ragged: sum of ceil(Q_b / 64))
   int64_t max_points;     // largest cloud (sizes the box reduction grid)
+  int32_t ways;           // blocks sharing one query tile's walk (power of two <= 64)
+  int32_t ways_max;       // largest split the workspace is sized for
   unsigned long long* stats;  // optional device counters (super tests, tile tests, scans, waves)
   int64_t K;
   const void *q, *p, *sigma, *R;

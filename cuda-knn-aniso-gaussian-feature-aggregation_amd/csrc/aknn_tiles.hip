@@ -42,7 +42,36 @@ struct PrunedState {
   int32_t* tcnt;
   void* tmp;
   size_t tmp_bytes;
+  void* lkey;       // split walk: per (sorted query, way) top-K keys   [qrows][ways][K]
+  int32_t* lid;     //                                    and indices
+  void* bnd;        // split walk: shared per-query bound (ordered encoding) [qrows]
   size_t total;
+};
+
+// Order-preserving unsigned encodings of the key (for atomic min on the shared bound).
+template <typename T>
+struct OrdKey;
+template <>
+struct OrdKey<float> {
+  using U = unsigned;
+  static __device__ __forceinline__ U enc(float f) {
+    const U u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  }
+  static __device__ __forceinline__ float dec(U k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+  }
+};
+template <>
+struct OrdKey<double> {
+  using U = unsigned long long;
+  static __device__ __forceinline__ U enc(double f) {
+    const U u = static_cast<U>(__double_as_longlong(f));
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+  }
+  static __device__ __forceinline__ double dec(U k) {
+    return __longlong_as_double(static_cast<long long>((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+  }
 };
 
 // Neighbouring floats (IEEE bit stepping).
@@ -343,13 +372,18 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
     const T* __restrict__ rec, const float* __restrict__ hdr, const float* __restrict__ shdr,
     const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
+    int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd) {
+  using Ord = OrdKey<T>;
   const int lane = threadIdx.x;
   int64_t b, q0;
   // XCD-aware order of the (spatially sorted) query tiles: each XCD's L2 then holds
-  // the point tiles of one region
+  // the point tiles of one region.  The `ways` blocks of one query tile are adjacent,
+  // so they share an XCD (and the bound they exchange stays in its L2).
   const int64_t nblk = static_cast<int64_t>(gridDim.x) * gridDim.y;
-  const int64_t lin = xcd_block(static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x, nblk);
+  const int64_t lin0 = xcd_block(static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x, nblk);
+  const int64_t way = lin0 % ways;
+  const int64_t lin = lin0 / ways;
   if (qtiles) {
     const int64_t t = lin;
     int64_t lo = 0, hi = cl.B - 1;
@@ -360,8 +394,9 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
     b = lo;
     q0 = cl.q_begin(b) + (t - qtiles[b]) * 64;
   } else {
-    b = lin / gridDim.x;
-    q0 = cl.q_begin(b) + (lin % gridDim.x) * 64;
+    const int64_t per = gridDim.x / ways;  // query tiles per cloud
+    b = lin / per;
+    q0 = cl.q_begin(b) + (lin % per) * 64;
   }
   const int64_t qend = cl.q_end(b);
   const int64_t pos = q0 + lane;
@@ -406,19 +441,28 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
   const int64_t seed_t = lo;
   const int64_t seed_s = sb + (seed_t - tb) / kSuper;
 
+  // Split walk (ways > 1): this block owns the tiles t with (t - tb) % ways == way and
+  // prunes with thr = the smallest last-kept key any way has published for the query.
+  // Every way's list is a subset's top-KMAX, so its last key bounds the full top-KMAX
+  // from above: a tile skipped against thr holds no point of the final result, whatever
+  // the timing of the exchange.
+  T thr = __builtin_inf();
   unsigned n_super = 0, n_tile = 0, n_scan = 0, n_pts = 0;
   Outward so(sb, se, seed_s);
   for (int64_t sstep = 0; sstep < se - sb; ++sstep) {
     const int64_t s = so.next(sstep);
     ++n_super;
-    if (!__any(box_needed<T>(shdr + s * 8, qx, qy, qz, key[KMAX - 1], valid))) continue;
+    if (ways > 1 && valid)
+      thr = fmin(thr, Ord::dec(__hip_atomic_load(bnd + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+    if (!__any(box_needed<T>(shdr + s * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid))) continue;
     const int64_t t0 = tb + (s - sb) * kSuper;
     const int64_t t1 = min(te, t0 + kSuper);
     Outward to(t0, t1, s == seed_s ? seed_t : t0);
     for (int64_t tstep = 0; tstep < t1 - t0; ++tstep) {
       const int64_t t = to.next(tstep);
+      if ((t - tb) % ways != way) continue;
       ++n_tile;
-      if (!__any(box_needed<T>(hdr + t * 8, qx, qy, qz, key[KMAX - 1], valid))) continue;
+      if (!__any(box_needed<T>(hdr + t * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid))) continue;
       ++n_scan;
       const int cnt = tcnt[t];
       n_pts += cnt;
@@ -440,15 +484,32 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
                                      r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
         if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
       }
+      if (ways > 1 && valid && key[KMAX - 1] < thr) {  // publish an improved bound
+        const T mine = key[KMAX - 1];
+        const typename Ord::U old = atomicMin(bnd + pos, Ord::enc(mine));
+        thr = fmin(mine, Ord::dec(old));
+      }
     }
   }
 
   if (valid) {
+    if (ways == 1) {
 #pragma unroll
-    for (int s = 0; s < KMAX; ++s) {
-      if (s < K) {
-        dists[row * K + s] = key[s];
-        idxs[row * K + s] = static_cast<int64_t>(id[s]);
+      for (int s = 0; s < KMAX; ++s) {
+        if (s < K) {
+          dists[row * K + s] = key[s];
+          idxs[row * K + s] = static_cast<int64_t>(id[s]);
+        }
+      }
+    } else {
+      T* lk = lkey + (pos * ways + way) * K;
+      int32_t* li = lid + (pos * ways + way) * K;
+#pragma unroll
+      for (int s = 0; s < KMAX; ++s) {
+        if (s < K) {
+          lk[s] = key[s];
+          li[s] = id[s];
+        }
       }
     }
   }
@@ -509,15 +570,93 @@ __global__ void super_build_kernel(Clouds cl, const int64_t* __restrict__ ptiles
   o[7] = 0.f;
 }
 
+template <typename U>
+__global__ void bound_fill_kernel(U* __restrict__ bnd, int64_t n, U v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) bnd[i] = v;
+}
+
+// Split walk: merge the `ways` per-way top-K lists of each sorted query position.
+// A wave holds 64 / ways queries, one lane per (query, way); a butterfly over the
+// ways merges the lists (disjoint tile sets, so no duplicates), lane way 0 writes.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(64) void pruned_merge_kernel(const T* __restrict__ lkey,
+                                                          const int32_t* __restrict__ lid,
+                                                          const uint32_t* __restrict__ qperm,
+                                                          int64_t qrows, int ways, int K,
+                                                          T* __restrict__ dists,
+                                                          int64_t* __restrict__ idxs) {
+  const int lane = threadIdx.x;
+  const int per = 64 / ways;
+  const int w = lane % ways;
+  const int64_t pos = static_cast<int64_t>(blockIdx.x) * per + lane / ways;
+  const bool ok = pos < qrows;
+  T key[KMAX];
+  int id[KMAX];
+#pragma unroll
+  for (int s = 0; s < KMAX; ++s) {
+    key[s] = __builtin_inf();
+    id[s] = kTileSentinel;
+  }
+  if (ok) {
+    const T* lk = lkey + (pos * ways + w) * K;
+    const int32_t* li = lid + (pos * ways + w) * K;
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+      if (s < K) {
+        key[s] = lk[s];
+        id[s] = li[s];
+      }
+    }
+  }
+  for (int off = 1; off < ways; off <<= 1) {
+    T pk[KMAX];
+    int pi[KMAX];
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+      pk[s] = __shfl_xor(key[s], off, 64);
+      pi[s] = __shfl_xor(id[s], off, 64);
+    }
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s)
+      if ((pk[s] < key[KMAX - 1]) | ((pk[s] == key[KMAX - 1]) & (pi[s] < id[KMAX - 1])))
+        lex_insert<KMAX>(key, id, pk[s], pi[s]);
+  }
+  if (ok && w == 0) {
+    const int64_t row = qperm[pos];
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+      if (s < K) {
+        dists[row * K + s] = key[s];
+        idxs[row * K + s] = static_cast<int64_t>(id[s]);
+      }
+    }
+  }
+}
+
 template <typename T, int KMAX, int KEY>
 static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+  using U = typename OrdKey<T>::U;
+  const int ways = L.ways > 1 ? L.ways : 1;
+  if (ways > 1) {
+    const U inf_enc = sizeof(T) == 4 ? U(0xff800000u) : U(0xfff0000000000000ull);  // enc(+inf)
+    hipLaunchKernelGGL((bound_fill_kernel<U>), dim3(static_cast<unsigned>((L.qrows + 255) / 256)), dim3(256),
+                       0, st, static_cast<U*>(S.bnd), L.qrows, inf_enc);
+  }
   dim3 grid;
-  if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles), 1);
-  else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64), static_cast<unsigned>(L.cl.B));
+  if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles * ways), 1);
+  else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64 * ways), static_cast<unsigned>(L.cl.B));
   hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY>), grid, dim3(64), 0, st, static_cast<const T*>(L.q),
                      L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp, S.stp,
                      static_cast<const T*>(S.rec), S.hdr, S.shdr, S.tcnt, static_cast<int>(L.K),
-                     static_cast<T*>(L.dists), L.idxs, L.stats);
+                     static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
+                     static_cast<U*>(S.bnd));
+  if (ways > 1) {
+    const int per = 64 / ways;
+    hipLaunchKernelGGL((pruned_merge_kernel<T, KMAX>), dim3(static_cast<unsigned>((L.qrows + per - 1) / per)),
+                       dim3(64), 0, st, static_cast<const T*>(S.lkey), S.lid, S.qperm, L.qrows, ways,
+                       static_cast<int>(L.K), static_cast<T*>(L.dists), L.idxs);
+  }
   return hipGetLastError();
 }
 
@@ -571,6 +710,12 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
   S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
+  if (L.ways > 1) {
+    const size_t lists = static_cast<size_t>(L.qrows) * L.ways * L.K;
+    S.lkey = c.take(lists * el);
+    S.lid = reinterpret_cast<int32_t*>(c.take(lists * 4));
+    S.bnd = c.take(static_cast<size_t>(L.qrows) * el);
+  }
   S.total = c.used;
   return S;
 }
@@ -587,6 +732,7 @@ hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes) {
   if (e != hipSuccess) return e;
   PrunedLaunch probe = L;
   probe.ws = nullptr;
+  probe.ways = L.ways_max > L.ways ? L.ways_max : L.ways;
   *bytes = carve(probe, tmp).total;
   return hipSuccess;
 }

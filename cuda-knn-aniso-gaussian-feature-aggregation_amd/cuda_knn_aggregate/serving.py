@@ -38,6 +38,7 @@ class Request:
     cloud: int
     q: torch.Tensor                  # [Q, 3] host (pinned) or device
     arrival: float = 0.0             # seconds on the server clock
+    started: float = -1.0            # launch of the batch that served it
     done: float = -1.0
     f_out: Optional[torch.Tensor] = None
     normal: Optional[torch.Tensor] = None
@@ -59,9 +60,11 @@ class _Batch:
 
 class Server:
     def __init__(self, clouds, K, v, *, device="cuda:0", streams=4, hbm_budget=2 << 30,
-                 max_batch_queries=131072, tune=None, key="aniso"):
+                 max_batch_queries=131072, tune=None, key="aniso", on_done=None):
         """clouds: list of (p [P,3], f [P,F], sigma [P,3], R [P,3,3]) tensors (moved to
-        ``device`` and kept resident); v: [F] read-out for the normal."""
+        ``device`` and kept resident); v: [F] read-out for the normal.  ``on_done(req)``
+        is called as each request completes (a consumer that takes the results and
+        drops them lets their pinned host buffers return to the allocator's cache)."""
         self.device = torch.device(device)
         self.K, self.key, self.tune = int(K), key, tune
         self.clouds = []
@@ -76,9 +79,9 @@ class Server:
         self.inflight: List[_Batch] = []
         self.in_use = 0
         self.waiting: collections.deque = collections.deque()
-        self.completed: List[Request] = []
         self.t0 = time.perf_counter()
         self.batches = 0
+        self.on_done = on_done
 
     # ------------------------------------------------------------------ clock
     def now(self) -> float:
@@ -108,6 +111,9 @@ class Server:
     def _launch(self, reqs: List[Request], slot: int, nbytes: int) -> None:
         p, f, s, R = self.clouds[reqs[0].cloud]
         st = self.streams[slot]
+        t = self.now()
+        for r in reqs:
+            r.started = t
         with torch.cuda.stream(st):
             qs = [r.q.to(self.device, non_blocking=True) for r in reqs]
             q = torch.cat(qs, 0)[None].contiguous()
@@ -162,7 +168,8 @@ class Server:
                     r.normal = n_host[off:off + m]
                     r.done = t
                     off += m
-                    self.completed.append(r)
+                    if self.on_done is not None:
+                        self.on_done(r)
                 self.free_slots.append(b.slot)
                 self.in_use -= b.bytes
                 n += 1
@@ -175,6 +182,30 @@ class Server:
         """One non-blocking scheduling round: reap finished batches, admit new ones."""
         self._reap()
         self._form_batches()
+
+    def warm_up(self, repeats: int = 2) -> None:
+        """Serve batches of every power-of-two size up to ``max_batch_queries`` on every
+        stream, so that the device and pinned-host caching allocators hold blocks of
+        each size class before timed serving starts."""
+        sizes = []
+        n = 256
+        while n < self.max_batch_queries:
+            sizes.append(n)
+            n *= 2
+        sizes.append(self.max_batch_queries)
+        keep = self.on_done
+        self.on_done = lambda r: (setattr(r, "f_out", None), setattr(r, "normal", None))
+        try:
+            for _ in range(repeats):
+                for n in sizes:
+                    for c in range(min(len(self.clouds), len(self.streams))):
+                        p = self.clouds[c][0][0]
+                        q = p[torch.arange(n, device=p.device) % p.shape[0]].cpu().pin_memory()
+                        self.submit(Request(c, q))
+                        self.poll()
+                    self.drain()
+        finally:
+            self.on_done = keep
 
     def drain(self) -> None:
         while self.waiting or self.inflight:

@@ -92,7 +92,9 @@ typedef struct aknn_dims {
 /* Tunables of the neighbour search (0 = library default for the shape). */
 typedef struct aknn_tune {
   int32_t queries_per_lane; /* brute: 1, 2 or 4 queries held per lane                  */
-  int32_t point_split;      /* brute: 1, 2, 4 or 8 waves splitting one tile's points   */
+  int32_t point_split;      /* brute: 1, 2, 4 or 8 waves splitting one tile's points;
+                               tiles: blocks sharing one query tile's walk (a power of
+                               two, at most the library's own choice for the shape)    */
   int32_t search;           /* AKNN_SEARCH_*                                            */
   int32_t reserved;
   void* stats;              /* tiles: optional device uint64[5] the search adds its counts

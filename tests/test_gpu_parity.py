@@ -125,6 +125,25 @@ def test_knn_tiles_degenerate_scales(cuda):
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+@pytest.mark.parametrize("ways", [1, 2, 4, 16, 64])
+def test_knn_tiles_split_walk(cuda, ways, key):
+    """Few, scattered queries: each query tile's walk split over `ways` blocks that
+    share a pruning bound; same bits as the oracle whatever the split (ways above the
+    library's cap for the shape are lowered to it)."""
+    b = workloads.clustered_batch(Q=500, P=30000, K=16, F=4, seed=11)
+    g = torch.Generator().manual_seed(12)
+    b.q[0] = torch.rand(500, 3, generator=g) * 1.2 - 0.1      # spread over the whole cloud
+    b.sigma[0, 100:140, 1] = 0.0
+    b.sigma[0, 400:420, 2] = -0.01
+    for t in (b.p, b.sigma, b.R):
+        t[0, 20000:20400] = t[0, 1000:1400]                   # exact duplicates (ties)
+    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": "tiles", "point_split": ways})
+    rd, ri = oracle_knn(b, key)
+    assert torch.equal(idxs.cpu(), ri)
+    assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
+
+
 @pytest.mark.parametrize("qpt", [1, 2, 4])
 @pytest.mark.parametrize("split", [1, 2, 4, 8])
 def test_knn_launch_shapes(cuda, qpt, split):
