@@ -247,6 +247,9 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
   if (porder) *porder = nullptr;
   if (pick_search(d, tune) == AKNN_SEARCH_TILES) {
     aknn::PrunedLaunch PL = pruned_launch(d);
+    if (PL.ntiles >= (int64_t(1) << 30))
+      return fail(AKNN_E_UNSUPPORTED, "%lld point tiles exceed the tiled search's 2^30 (use the brute scan)",
+                  static_cast<long long>(PL.ntiles));
     PL.q = q;
     PL.p = p;
     PL.sigma = sigma;

@@ -11,8 +11,10 @@
 //      FP32 key the scan computes satisfies   key(q, n) >= mu_n * |q - p_n|^2.
 //   3. One wave = 64 spatially sorted queries.  It visits the cloud's tiles outward
 //      from the tile nearest its first query; a tile is scanned only if some lane has
-//      mu_tile * dist^2(q, box) <= its current K-th key.  Skipped points provably have
-//      keys above every lane's K-th, so the kept set and order are those of a full scan.
+//      mu_tile * dist^2(q, box) <= its current K-th key, and within it only the
+//      16-record quarters whose own box and mu pass the same test.  Skipped points
+//      provably have keys above every lane's K-th, so the kept set and order are those
+//      of a full scan.
 //
 // mu_n (point with M = R^T diag(s) R): for the exact D = d^T M d >= lmin(M) |d|^2, and
 // for the scan's FP32 evaluation (one rounding per op, unit roundoff u, g = 4u/(1-4u)):
@@ -38,7 +40,7 @@ struct PrunedState {
   uint32_t *pperm, *qperm;
   int64_t *ptp, *qtp, *stp;
   void* rec;
-  float *hdr, *shdr;
+  float *hdr, *shdr, *sub;  // tile, super-tile and sub-tile (16 records) headers
   int32_t* tcnt;
   void* tmp;
   size_t tmp_bytes;
@@ -235,13 +237,14 @@ __device__ double point_mu(const T* r9, const T* s3) {
 
 // ------------------------------------------------------------------ tile build
 // One wave per tile.  rec: [tiles*kTile][16] records; hdr: [tiles][8] =
-// {lo x y z, hi x y z, mu (float, rounded down; < 0 = always scan), first code (bits)}.
+// {lo x y z, hi x y z, mu (float, rounded down; < 0 = always scan), first code (bits)};
+// sub: [tiles][4][8] = the same box and mu (last value unused) of each 16-record quarter.
 template <typename T, int KEY>
 __global__ __launch_bounds__(64) void tile_build_kernel(
     const T* __restrict__ p, const T* __restrict__ sigma, const T* __restrict__ R, Clouds cl,
     const int64_t* __restrict__ tile_prefix, const uint32_t* __restrict__ perm,
     const uint64_t* __restrict__ sorted_keys, T* __restrict__ rec, float* __restrict__ hdr,
-    int32_t* __restrict__ tile_cnt) {
+    float* __restrict__ sub, int32_t* __restrict__ tile_cnt) {
   const int64_t t = blockIdx.x;
   const int lane = threadIdx.x;
   int64_t b = 0;
@@ -285,19 +288,15 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     mu = point_mu<T, KEY>(v + 3, v + 12);
     if (!(mu >= 0.0)) bad = true;
   } else {
-    for (int c = 0; c < 16; ++c) out[c] = T(0);
+    // padding record: NaN position, so every key evaluated against it is NaN and it
+    // never enters a top-K list (the scan reads whole 16-record quarters)
+    for (int c = 0; c < 16; ++c) out[c] = c < 3 ? static_cast<T>(__builtin_nan("")) : T(0);
+    if constexpr (sizeof(T) == 4) out[15] = __int_as_float(kTileSentinel);
+    else out[15] = __longlong_as_double(static_cast<long long>(kTileSentinel));
   }
-  // wave reductions (64 lanes)
-  for (int off = 32; off > 0; off >>= 1) {
-    for (int c = 0; c < 3; ++c) {
-      lo3[c] = fminf(lo3[c], __shfl_xor(lo3[c], off, 64));
-      hi3[c] = fmaxf(hi3[c], __shfl_xor(hi3[c], off, 64));
-    }
-    mu = fmin(mu, __shfl_xor(mu, off, 64));
-    bad = bad | (__shfl_xor(static_cast<int>(bad), off, 64) != 0);
-  }
-  if (lane == 0) {
-    float* h = hdr + t * 8;
+  // Reductions over groups of 16 lanes (the sub-tiles), then over the wave (the tile).
+  // min / max are exact, so the grouping does not change any value.
+  auto write_box = [&](float* h) {
     for (int c = 0; c < 3; ++c) {
       h[c] = lo3[c];
       h[3 + c] = hi3[c];
@@ -309,6 +308,23 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
       if (static_cast<double>(muf) > m) muf = next_down(muf);
     }
     h[6] = muf;
+  };
+  for (int off = 1; off < 64; off <<= 1) {
+    for (int c = 0; c < 3; ++c) {
+      lo3[c] = fminf(lo3[c], __shfl_xor(lo3[c], off, 64));
+      hi3[c] = fmaxf(hi3[c], __shfl_xor(hi3[c], off, 64));
+    }
+    mu = fmin(mu, __shfl_xor(mu, off, 64));
+    bad = bad | (__shfl_xor(static_cast<int>(bad), off, 64) != 0);
+    if (off == 8 && (lane & 15) == 0) {
+      float* h = sub + (t * 4 + lane / 16) * 8;
+      write_box(h);
+      h[7] = 0.f;
+    }
+  }
+  if (lane == 0) {
+    float* h = hdr + t * 8;
+    write_box(h);
     h[7] = __uint_as_float(static_cast<uint32_t>(sorted_keys[i0] & 0xffffffffu));
     tile_cnt[t] = cnt;
   }
@@ -333,6 +349,59 @@ __device__ __forceinline__ void lex_insert(T (&k)[N], int (&v)[N], T c, int ci) 
   }
 }
 
+// Same result as lex_insert, as a chain of compare-and-swap steps: only one compare
+// mask is live at a time, which keeps the scalar registers free for the records.
+template <int N, typename T>
+__device__ __forceinline__ void bubble_insert(T (&k)[N], int (&v)[N], T c, int ci) {
+  T ck = c;
+  int cv = ci;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const bool sw = (ck < k[i]) | ((ck == k[i]) & (cv < v[i]));
+    const T tk = k[i];
+    const int tv = v[i];
+    k[i] = sw ? ck : tk;
+    v[i] = sw ? cv : tv;
+    ck = sw ? tk : ck;
+    cv = sw ? tv : cv;
+  }
+}
+
+// One 16-value point record, read with wave-uniform addresses (scalar loads).
+template <typename T>
+struct Rec16 {
+  T v[16];
+};
+template <typename T>
+__device__ __forceinline__ Rec16<T> load_rec16(const T* __restrict__ r) {
+  Rec16<T> o;
+  if constexpr (sizeof(T) == 4) {
+    const float4* r4 = reinterpret_cast<const float4*>(r);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 x = r4[i];
+      o.v[4 * i] = x.x;
+      o.v[4 * i + 1] = x.y;
+      o.v[4 * i + 2] = x.z;
+      o.v[4 * i + 3] = x.w;
+    }
+  } else {
+    const double2* r2 = reinterpret_cast<const double2*>(r);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double2 x = r2[i];
+      o.v[2 * i] = x.x;
+      o.v[2 * i + 1] = x.y;
+    }
+  }
+  return o;
+}
+template <typename T, int KEY>
+__device__ __forceinline__ T rec_key(T qx, T qy, T qz, const Rec16<T>& r) {
+  return pair_key<T, KEY>(qx, qy, qz, r.v[0], r.v[1], r.v[2], r.v[3], r.v[4], r.v[5], r.v[6], r.v[7],
+                          r.v[8], r.v[9], r.v[10], r.v[11], r.v[12], r.v[13], r.v[14]);
+}
+
 template <typename T>
 __device__ __forceinline__ int rec_index(T v) {
   if constexpr (sizeof(T) == 4) return __float_as_int(v);
@@ -355,9 +424,9 @@ __device__ __forceinline__ bool box_needed(const float* __restrict__ h, T qx, T 
 
 // Outward walk over [lo, hi) from c: c, c+1, c-1, c+2, c-2, ... (each index once).
 struct Outward {
-  int64_t up, down, lo, hi;
-  __device__ Outward(int64_t lo_, int64_t hi_, int64_t c) : up(c), down(c - 1), lo(lo_), hi(hi_) {}
-  __device__ __forceinline__ int64_t next(int64_t step) {
+  int up, down, lo, hi;
+  __device__ Outward(int lo_, int hi_, int c) : up(c), down(c - 1), lo(lo_), hi(hi_) {}
+  __device__ __forceinline__ int next(int step) {
     const bool take_up = (up < hi) && ((step & 1) == 0 || down < lo);
     return take_up ? up++ : down--;
   }
@@ -366,13 +435,13 @@ struct Outward {
 // grid.x = query tiles of 64 sorted queries (ragged: via the query tile prefix),
 // grid.y = clouds (dense).  Two-level walk: super-tiles of kSuper tiles, then tiles.
 template <typename T, int KMAX, int KEY>
-__global__ __launch_bounds__(64) void knn_pruned_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void knn_pruned_kernel(
     const T* __restrict__ q, Clouds cl, const int64_t* __restrict__ qtiles,
     const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys,
     const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
     const T* __restrict__ rec, const float* __restrict__ hdr, const float* __restrict__ shdr,
-    const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
-    unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
+    const float* __restrict__ sub, const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists,
+    int64_t* __restrict__ idxs, unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
     int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd) {
   using Ord = OrdKey<T>;
   const int lane = threadIdx.x;
@@ -419,27 +488,30 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
   }
 
   // this cloud's tiles [tb, te) and super-tiles [sb, se)
-  int64_t tb, te, sb, se;
+  // tile and super-tile numbers fit in 32 bits (the host checks the tile count); the
+  // walk state is kept narrow to leave scalar registers for the point records
+  int tb, te, sb, se;
   if (cl.p_off) {
-    tb = ptiles[b];
-    te = ptiles[b + 1];
-    sb = stiles[b];
-    se = stiles[b + 1];
+    tb = static_cast<int>(ptiles[b]);
+    te = static_cast<int>(ptiles[b + 1]);
+    sb = static_cast<int>(stiles[b]);
+    se = static_cast<int>(stiles[b + 1]);
   } else {
-    tb = b * ptiles[1];
-    te = tb + ptiles[1];
-    sb = b * stiles[1];
-    se = sb + stiles[1];
+    tb = static_cast<int>(b * ptiles[1]);
+    te = tb + static_cast<int>(ptiles[1]);
+    sb = static_cast<int>(b * stiles[1]);
+    se = sb + static_cast<int>(stiles[1]);
   }
   // seed: last tile whose first code <= the wave's first query code (scalar search)
   const uint32_t code0 = static_cast<uint32_t>(qkeys[q0] & 0xffffffffu);
-  int64_t lo = tb, hi = te - 1;
+  int lo = tb, hi = te - 1;
   while (lo < hi) {
-    const int64_t mid = (lo + hi + 1) >> 1;
-    if (__float_as_uint(hdr[mid * 8 + 7]) <= code0) lo = mid; else hi = mid - 1;
+    const int mid = (lo + hi + 1) >> 1;
+    if (__float_as_uint(hdr[static_cast<int64_t>(mid) * 8 + 7]) <= code0) lo = mid; else hi = mid - 1;
   }
-  const int64_t seed_t = lo;
-  const int64_t seed_s = sb + (seed_t - tb) / kSuper;
+  const int seed_t = lo;
+  const int seed_s = sb + (seed_t - tb) / kSuper;
+  const int iway = static_cast<int>(way);
 
   // Split walk (ways > 1): this block owns the tiles t with (t - tb) % ways == way and
   // prunes with thr = the smallest last-kept key any way has published for the query.
@@ -449,40 +521,61 @@ __global__ __launch_bounds__(64) void knn_pruned_kernel(
   T thr = __builtin_inf();
   unsigned n_super = 0, n_tile = 0, n_scan = 0, n_pts = 0;
   Outward so(sb, se, seed_s);
-  for (int64_t sstep = 0; sstep < se - sb; ++sstep) {
-    const int64_t s = so.next(sstep);
+  for (int sstep = 0; sstep < se - sb; ++sstep) {
+    const int s = so.next(sstep);
     ++n_super;
     if (ways > 1 && valid)
       thr = fmin(thr, Ord::dec(__hip_atomic_load(bnd + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-    if (!__any(box_needed<T>(shdr + s * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid))) continue;
-    const int64_t t0 = tb + (s - sb) * kSuper;
-    const int64_t t1 = min(te, t0 + kSuper);
+    if (!__any(box_needed<T>(shdr + static_cast<int64_t>(s) * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid)))
+      continue;
+    const int t0 = tb + (s - sb) * kSuper;
+    const int t1 = min(te, t0 + kSuper);
     Outward to(t0, t1, s == seed_s ? seed_t : t0);
-    for (int64_t tstep = 0; tstep < t1 - t0; ++tstep) {
-      const int64_t t = to.next(tstep);
-      if ((t - tb) % ways != way) continue;
+    for (int tstep = 0; tstep < t1 - t0; ++tstep) {
+      const int t = to.next(tstep);
+      if ((t - tb) % ways != iway) continue;
       ++n_tile;
-      if (!__any(box_needed<T>(hdr + t * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid))) continue;
+      if (!__any(box_needed<T>(hdr + static_cast<int64_t>(t) * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid)))
+        continue;
       ++n_scan;
       const int cnt = tcnt[t];
-      n_pts += cnt;
-      const T* __restrict__ base = rec + t * (kTile * 16);
-      int j = 0;
-      for (; j + 2 <= cnt; j += 2) {  // two records per trip: both loads issue together
-        const T* r = base + j * 16;
-        const T* r2 = r + 16;
-        const T c = pair_key<T, KEY>(qx, qy, qz, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7],
-                                     r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
-        const T c2 = pair_key<T, KEY>(qx, qy, qz, r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6],
-                                      r2[7], r2[8], r2[9], r2[10], r2[11], r2[12], r2[13], r2[14]);
-        if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
-        if (c2 <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c2, rec_index(r2[15]));
-      }
-      if (j < cnt) {
-        const T* r = base + j * 16;
-        const T c = pair_key<T, KEY>(qx, qy, qz, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7],
-                                     r[8], r[9], r[10], r[11], r[12], r[13], r[14]);
-        if (c <= key[KMAX - 1]) lex_insert<KMAX>(key, id, c, rec_index(r[15]));
+      // the tile's four sub-tiles of 16 records: scan those some lane still needs
+      const float* __restrict__ sh = sub + static_cast<int64_t>(t) * 32;
+      const T tt = fmin(key[KMAX - 1], thr);
+      unsigned m = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if ((16 * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, tt, valid))) m |= 1u << u;
+      if (m == 0) continue;
+      const int total = 16 * __builtin_popcount(m);
+      n_pts += total;
+      const T* __restrict__ base = rec + static_cast<int64_t>(t) * (kTile * 16);
+      // The needed quarters' records a pair at a time, as one stream; the next pair's
+      // scalar loads are issued before the current pair is evaluated.  Scalar loads
+      // complete out of order, so each wait is for everything issued: with this order
+      // it falls after the evaluation.  Quarters are whole (padding records are NaN).
+      int u = __builtin_ctz(m), jj = 0;
+      Rec16<T> a0 = load_rec16(base + u * 256);
+      Rec16<T> a1 = load_rec16(base + u * 256 + 16);
+#pragma unroll 2
+      for (int j = 0; j < total; j += 2) {
+        int un = u, jn = jj + 2;
+        if (jn == 16) {
+          const unsigned rest = m & ~((2u << u) - 1u);
+          un = rest ? __builtin_ctz(rest) : u;
+          jn = rest ? 0 : 14;  // past the end: reload the last pair (not evaluated)
+        }
+        const T* nx = base + (un * 16 + jn) * 16;
+        const Rec16<T> b0 = load_rec16(nx);
+        const Rec16<T> b1 = load_rec16(nx + 16);
+        const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
+        const T c1 = rec_key<T, KEY>(qx, qy, qz, a1);
+        if (c0 <= key[KMAX - 1]) bubble_insert<KMAX>(key, id, c0, rec_index(a0.v[15]));
+        if (c1 <= key[KMAX - 1]) bubble_insert<KMAX>(key, id, c1, rec_index(a1.v[15]));
+        a0 = b0;
+        a1 = b1;
+        u = un;
+        jj = jn;
       }
       if (ways > 1 && valid && key[KMAX - 1] < thr) {  // publish an improved bound
         const T mine = key[KMAX - 1];
@@ -648,7 +741,7 @@ static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, h
   else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64 * ways), static_cast<unsigned>(L.cl.B));
   hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY>), grid, dim3(64), 0, st, static_cast<const T*>(L.q),
                      L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp, S.stp,
-                     static_cast<const T*>(S.rec), S.hdr, S.shdr, S.tcnt, static_cast<int>(L.K),
+                     static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
                      static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
                      static_cast<U*>(S.bnd));
   if (ways > 1) {
@@ -707,6 +800,7 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
   S.rec = c.take(L.ntiles * kTile * 16 * el);
   S.hdr = reinterpret_cast<float*>(c.take(L.ntiles * 8 * 4));
   S.shdr = reinterpret_cast<float*>(c.take(L.nsuper * 8 * 4));
+  S.sub = reinterpret_cast<float*>(c.take(L.ntiles * 4 * 8 * 4));
   S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
@@ -801,7 +895,7 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
   }
   hipLaunchKernelGGL((tile_build_kernel<T, KEY>), dim3(static_cast<unsigned>(L.ntiles)), dim3(64), 0, st,
                      p, static_cast<const T*>(L.sigma), static_cast<const T*>(L.R), L.cl, S.ptp, S.pperm,
-                     S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.tcnt);
+                     S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
   hipLaunchKernelGGL(super_build_kernel, dim3(static_cast<unsigned>((L.nsuper + bs - 1) / bs)), dim3(bs), 0, st,
                      L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr);
   return hipGetLastError();
