@@ -50,7 +50,19 @@ __device__ __forceinline__ void pstatic_for(Fn&& fn) {
   }
 }
 
-template <typename T, int VEC, int CH, int KEY, bool SECOND>
+// Sum over the G 16-lane groups of a wave that share one point row (fixed order:
+// (g0 + g1) + (g2 + g3); every group ends with the same bits).
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+  if constexpr (G >= 2) v = v + __shfl_xor(v, 16, 64);
+  if constexpr (G >= 4) v = v + __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// G = 16-lane groups per point row (1: four rows per wave; 4: one row per wave, its
+// entries dealt to the groups in turn).  Fewer rows in flight keep the gathered query
+// rows of the rows being processed within L2.
+template <typename T, int VEC, int CH, int KEY, bool SECOND, int G>
 __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const T* __restrict__ q = static_cast<const T*>(L.q);
   const T* __restrict__ f = static_cast<const T*>(L.f);
@@ -62,8 +74,9 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int K = static_cast<int>(L.K);
 
   const int r = threadIdx.x & 15;
+  const int gq = (threadIdx.x >> 4) % G;
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t gi = blk * 16 + (threadIdx.x >> 4);
+  const int64_t gi = blk * (16 / G) + (threadIdx.x >> 4) / G;
   const bool row_ok = gi < L.rows;
   const int64_t gc = row_ok ? gi : L.rows - 1;
   // spatial processing order (when given): neighbouring points share queries, so the
@@ -98,8 +111,8 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
 #pragma unroll
   for (int i = 0; i < 15; ++i) acc[i] = T(0);
 
-  for (int eb = e0; eb < e1; eb += 16) {
-    const int e = eb + r;
+  for (int eb = e0; eb < e1; eb += 16 * G) {
+    const int e = eb + 16 * gq + r;
     const bool have = e < e1;
     const int entry = have ? ent[e] : 0;
     const int64_t qrow = have ? static_cast<int64_t>(entry / K) : 0;
@@ -203,8 +216,14 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   }
 
 #pragma unroll
-  for (int i = 0; i < 15; ++i) acc[i] = row_sum(acc[i]);
-  if (!row_ok) return;
+  for (int i = 0; i < 15; ++i) acc[i] = group_sum<G>(row_sum(acc[i]));
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) dfa[c][i] = group_sum<G>(dfa[c][i]);
+  }
+  if (!row_ok || gq != 0) return;
   T* dp = static_cast<T*>(L.dp);
   T* ds = static_cast<T*>(L.dsigma);
   T* dR = static_cast<T*>(L.dR);
@@ -246,15 +265,19 @@ static int pick_vec_points(const PointLaunch& L) {
   return 1;
 }
 
+constexpr int kPointGroups = 4;  // 16-lane groups per point row
+
 template <typename T, int VEC, int KEY, bool SECOND>
 static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
+  constexpr int G = kPointGroups;
   const int64_t per = 16 * VEC;
   const int64_t ch = (L.F + per - 1) / per;
-  const dim3 grid(static_cast<unsigned>((L.rows + 15) / 16)), block(256);
-  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND>), grid, block, 0, st, L);
-  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND>), grid, block, 0, st, L);
-  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND>), grid, block, 0, st, L);
-  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND>), grid, block, 0, st, L);
+  const int64_t rows_per_block = 16 / G;
+  const dim3 grid(static_cast<unsigned>((L.rows + rows_per_block - 1) / rows_per_block)), block(256);
+  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G>), grid, block, 0, st, L);
+  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G>), grid, block, 0, st, L);
+  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G>), grid, block, 0, st, L);
+  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G>), grid, block, 0, st, L);
   else return hipErrorNotSupported;
   return hipGetLastError();
 }
