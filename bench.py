@@ -39,6 +39,48 @@ WORKLOADS = {
 }
 
 
+# rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE per launch, scaled by a plain-copy
+# calibration) of the same workload, committed beside the kernel statistics
+TRAFFIC_PROFILE = {"clustered": os.path.join(ROOT, "profiles", "hbm_traffic_clustered_1M_64K.md")}
+
+
+def measured_traffic(workload):
+    """{kernel: (bytes read, bytes written) per launch} from the committed profile, or {}."""
+    path = TRAFFIC_PROFILE.get(workload)
+    if not path or not os.path.exists(path):
+        return {}
+    out, inside = {}, False
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("```traffic"):
+            inside = True
+        elif line.startswith("```"):
+            inside = False
+        elif inside and line:
+            name, rd, wr = line.rsplit(maxsplit=2)
+            out[name] = (int(rd), int(wr))
+    return out
+
+
+def stream_copy_gbs(dev, n=1 << 28, reps=5):
+    """Device-to-device copy bandwidth on this GPU (1 GiB each way), GB/s."""
+    import torch
+
+    src = torch.empty(n, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2 * n * 4 * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    return gbs
+
+
 def make_batch(name, seed=0):
     from cuda_knn_aggregate import workloads
 
@@ -324,25 +366,38 @@ def main():
     else:
         search_stats = None
 
+    traffic = measured_traffic(args.workload)
+
+    def traffic_of(kname):
+        t = traffic.get(kname)
+        return None if t is None else t[0] + t[1]
+
     def roof(kname):
         avg_s = kernels[kname]["avg_ms"] * 1e-3
         if kname in pairs:
             flops = KEY_FLOP["aniso"] * pairs[kname]
             ach = flops / avg_s / 1e12
             return {"kernel": kname, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                    "per_launch": f"{KEY_FLOP['aniso']} FLOP x {pairs[kname]} evaluated (query, point) pairs"}
+                    "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": traffic_of(kname),
+                    "per_launch": f"{KEY_FLOP['aniso']} FLOP x {pairs[kname]} evaluated (query, point) pairs",
+                    "unfused_valu_peak": FP32_PEAK_TFLOPS / 4,
+                    "frac_of_unfused_valu_peak": ach / (FP32_PEAK_TFLOPS / 4),
+                    "brute_equivalent_tflops": KEY_FLOP["aniso"] * B * Q * P / avg_s / 1e12}
         byt = algorithmic_bytes(kname, B, Q, P, F, K)
         ach = byt / avg_s / 1e9
         return {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "per_launch": f"{byt} algorithmic bytes"}
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic_of(kname), "per_launch": f"{byt} algorithmic bytes"}
 
     gather = [k for k in ("aggregate", "query_grad", "point_grad", "query_grad2", "point_grad2") if k in kernels]
     g_bytes = sum(algorithmic_bytes(k, B, Q, P, F, K) * kernels[k]["launches"] / args.steps for k in gather)
     g_s = sum(kernels[k]["ms_per_step"] for k in gather) * 1e-3
+    g_traffic = [traffic_of(k) for k in gather]
+    copy_gbs = stream_copy_gbs(dev)
     roof_agg = {"kernels": gather, "bound": "hbm", "achieved": g_bytes / g_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": g_bytes / g_s / 1e9 / HBM_PEAK_GBS, "ms_per_step": g_s * 1e3,
-                "bytes_per_query": g_bytes / (B * Q)}
+                "bytes_per_query": g_bytes / (B * Q),
+                "traffic_per_step": None if None in g_traffic else sum(g_traffic),
+                "stream_copy_gbs": copy_gbs, "frac_of_stream_copy": g_bytes / g_s / 1e9 / copy_gbs}
 
     out = {
         "metric": METRIC,
