@@ -477,6 +477,116 @@ int aknn_aggregate(const aknn_dims* d, const void* q, const void* p, const void*
   return AKNN_OK;
 }
 
+size_t aknn_iso_workspace_size(const aknn_dims* d) {
+  if (!d) return 0;
+  aknn_dims e = *d;
+  e.key = AKNN_KEY_EUCLIDEAN;
+  return aknn_forward_workspace_size(&e);
+}
+
+int aknn_iso_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
+                     const void* f, const void* scale, void* f_out, void* w_out, void* dists,
+                     int64_t* idxs, void* ws, size_t ws_bytes, void* stream) {
+  if (!d) return fail(AKNN_E_INVALID, "dims is NULL");
+  aknn_dims e = *d;
+  e.key = AKNN_KEY_EUCLIDEAN;
+  int rc = check_dims(&e, true);
+  if (rc) return rc;
+  if (query_rows(&e) == 0 || e.B == 0) return AKNN_OK;
+  if (!q || !p || !f || !scale || !f_out || !w_out || !dists || !idxs)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  const size_t need = aknn_iso_workspace_size(&e);
+  if (need == 0) return AKNN_E_LAUNCH;
+  if (!ws || ws_bytes < need) return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes, need);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  rc = run_knn(&e, tune, q, p, nullptr, nullptr, dists, idxs, ws, ws_bytes, st);
+  if (rc) return rc;
+  aknn::QueryLaunch L{};
+  L.dtype = e.dtype;
+  L.key = AKNN_KEY_EUCLIDEAN;
+  L.cl = clouds_of(&e);
+  L.rows = query_rows(&e);
+  L.F = e.F;
+  L.K = e.K;
+  L.q = q; L.p = p; L.f = f; L.sigma = scale;
+  L.idxs = idxs;
+  L.dists = dists;  // the scan's Euclidean keys
+  L.f_out = f_out;
+  L.w_out = w_out;
+  hipError_t err;
+  {
+    Span sp("aggregate_iso", st);
+    err = aknn::launch_aggregate_iso(L, st);
+  }
+  if (err != hipSuccess) return hip_fail(err, "aggregate_iso");
+  return AKNN_OK;
+}
+
+// Lookup workspace: the 1-NN scan's own workspace, then its key and index per query.
+static aknn_dims nn1_dims(const aknn_dims* d) {
+  aknn_dims e = *d;
+  e.K = 1;
+  e.key = AKNN_KEY_EUCLIDEAN;
+  return e;
+}
+
+size_t aknn_lookup_workspace_size(const aknn_dims* d) {
+  if (!d) return 0;
+  const aknn_dims e = nn1_dims(d);
+  const size_t scan = aknn_forward_workspace_size(&e);
+  if (scan == 0) return 0;
+  return align256(scan) + align256(static_cast<size_t>(query_rows(d)) * elem_size(d)) +
+         align256(static_cast<size_t>(query_rows(d)) * 8);
+}
+
+int aknn_lookup_forward(const aknn_dims* d, const void* q, const void* p, const void* f, const void* sigma,
+                        const void* R, const int64_t* knn_table, int64_t table_k, void* f_out, void* w_out,
+                        void* dists, int64_t* idxs, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (d->q_off) return fail(AKNN_E_UNSUPPORTED, "the table lookup takes dense batches (one [P, K] table)");
+  if (table_k < d->K) return fail(AKNN_E_INVALID, "table width %lld < K %lld", (long long)table_k, (long long)d->K);
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!q || !p || !f || !sigma || !R || !knn_table || !f_out || !w_out || !dists || !idxs)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  const size_t need = aknn_lookup_workspace_size(d);
+  if (need == 0) return AKNN_E_LAUNCH;
+  if (!ws || ws_bytes < need) return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes, need);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const aknn_dims e = nn1_dims(d);
+  const size_t scan = align256(aknn_forward_workspace_size(&e));
+  char* base = static_cast<char*>(ws);
+  void* d1 = base + scan;
+  int64_t* i1 = reinterpret_cast<int64_t*>(base + scan + align256(static_cast<size_t>(query_rows(d)) * elem_size(d)));
+  rc = run_knn(&e, nullptr, q, p, sigma, R, d1, i1, ws, scan, st);
+  if (rc) return rc;
+  hipError_t err;
+  {
+    Span sp("lookup_gather", st);
+    err = aknn::launch_lookup_gather(d1, i1, knn_table, table_k, query_rows(d), d->K, d->dtype, idxs, st);
+  }
+  if (err != hipSuccess) return hip_fail(err, "lookup_gather");
+  aknn::QueryLaunch L{};
+  L.dtype = d->dtype;
+  L.key = AKNN_KEY_ANISO;
+  L.cl = clouds_of(d);
+  L.rows = query_rows(d);
+  L.F = d->F;
+  L.K = d->K;
+  L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+  L.idxs = idxs;
+  L.dists = nullptr;  // anisotropic D recomputed for the table's neighbours
+  L.dists_out = dists;
+  L.f_out = f_out;
+  L.w_out = w_out;
+  {
+    Span sp("aggregate", st);
+    err = aknn::launch_aggregate(L, st);
+  }
+  if (err != hipSuccess) return hip_fail(err, "aggregate");
+  return AKNN_OK;
+}
+
 size_t aknn_csr_workspace_size(const aknn_dims* d) {
   if (check_dims(d, false) != AKNN_OK) return 0;
   size_t bytes = 0;

@@ -57,7 +57,9 @@ __device__ __forceinline__ double row_get(double v) {
                               static_cast<long long>(static_cast<unsigned>(lo)));
 }
 
-enum { kFwd = 0, kBwd1 = 1, kBwd2 = 2 };
+// kIso: the reference's isotropic forward (csrc/forward.cu:5-53): Euclidean keys,
+// w = exp(-(d^2 * s_n)) with one scale per point, w_out = sum of the K weights.
+enum { kFwd = 0, kBwd1 = 1, kBwd2 = 2, kIso = 3 };
 
 // Unrolled loop over the 16 slots of a batch (J must be a compile-time constant for
 // the swizzle pattern).
@@ -71,6 +73,7 @@ __device__ __forceinline__ void static_for(F&& fn) {
 
 template <typename T, int VEC, int CH, int KEY, int MODE>
 __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
+  constexpr bool kBack = MODE == kBwd1 || MODE == kBwd2;
   const T* __restrict__ q = static_cast<const T*>(L.q);
   const T* __restrict__ p = static_cast<const T*>(L.p);
   const T* __restrict__ f = static_cast<const T*>(L.f);
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   for (int c = 0; c < CH; ++c)
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[c][i] = T(0);
-  if constexpr (MODE != kFwd) {
+  if constexpr (kBack) {
     const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -129,6 +132,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   }
   T dqx = T(0), dqy = T(0), dqz = T(0);
   const bool want_dq = L.dq != nullptr;  // uniform: otherwise only d grad_f_out is made
+  T wsum = T(0);                         // kIso: weights summed in slot order
 
   for (int kb = 0; kb < K; kb += 16) {
     const int k = kb + r;
@@ -137,7 +141,21 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     T w = T(0), coef = T(0);
     Geo<T> geo{};
     Pt<T> pt{};
-    if constexpr (MODE == kFwd) {
+    if constexpr (MODE == kIso) {
+      if (have) {
+        T D;
+        if (L.dists) {  // the Euclidean scan's keys (same expression, same bits)
+          D = static_cast<const T*>(L.dists)[row * K + k];
+        } else {
+          const T* __restrict__ pn = p + (pb + n) * 3;
+          D = pair_key<T, AKNN_KEY_EUCLIDEAN>(qx, qy, qz, pn[0], pn[1], pn[2], T(0), T(0), T(0), T(0), T(0),
+                                              T(0), T(0), T(0), T(0), T(0), T(0), T(0));
+        }
+        if (row_ok && L.dists_out) static_cast<T*>(L.dists_out)[row * K + k] = D;
+        w = gexp(-(D * sg[pb + n]));
+      }
+      coef = w;
+    } else if constexpr (MODE == kFwd) {
       if (have) {
         T D;
         if (L.dists && KEY == AKNN_KEY_ANISO) {  // the scan's keys are the weights' D
@@ -177,6 +195,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) fv[J.value][c] = *reinterpret_cast<const VecT<T, VEC>*>(frow + fofs[c]);
     });
+    if constexpr (MODE == kIso) static_for<0, 16>([&](auto J) { wsum += cj[J.value]; });
     T part[16];
     static_for<0, 16>([&](auto J) {
       T dot = T(0);
@@ -186,13 +205,13 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
         for (int i = 0; i < VEC; ++i) {
           const T x = fok[c] ? fv[J.value][c].v[i] : T(0);
           if constexpr (MODE != kBwd1) acc[c][i] += cj[J.value] * x;
-          if constexpr (MODE != kFwd) dot += gsl[c][i] * x;
+          if constexpr (kBack) dot += gsl[c][i] * x;
         }
       }
       part[J.value] = dot;
     });
 
-    if (MODE != kFwd && want_dq) {
+    if (kBack && want_dq) {
       const T a = row_transpose_sum<T>(part, r);  // a for slot k
       if (have) {
         if constexpr (MODE == kBwd1) {
@@ -219,7 +238,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     }
   }
 
-  if (MODE != kFwd && want_dq) {
+  if (kBack && want_dq) {
     dqx = row_sum(dqx);
     dqy = row_sum(dqy);
     dqz = row_sum(dqz);
@@ -228,8 +247,11 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
       static_cast<T*>(L.dq)[row * 3 + r] = val;
     }
   }
+  if constexpr (MODE == kIso) {
+    if (row_ok && r == 0) static_cast<T*>(L.w_out)[row] = wsum;
+  }
   if constexpr (MODE != kBwd1) {
-    T* __restrict__ out = static_cast<T*>(MODE == kFwd ? L.f_out : L.d_grad_f_out);
+    T* __restrict__ out = static_cast<T*>(MODE == kBwd2 ? L.d_grad_f_out : L.f_out);
     if (row_ok && out) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
@@ -291,6 +313,41 @@ static hipError_t launch_q(const QueryLaunch& L, hipStream_t st) {
 }
 
 hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st) { return launch_q<kFwd>(L, st); }
+
+hipError_t launch_aggregate_iso(const QueryLaunch& L, hipStream_t st) {
+  if (L.dtype == AKNN_F32) return launch_q_vec<float, AKNN_KEY_EUCLIDEAN, kIso>(L, st);
+  return launch_q_vec<double, AKNN_KEY_EUCLIDEAN, kIso>(L, st);
+}
+
+// ------------------------------------------------------------------ lookup
+// The reference's table lookup (csrc/forward.cu:113-183): the query's Euclidean
+// nearest point n1 (first index reaching the minimum below 1e10, else 0) selects row
+// n1 of the [P, table_k] neighbour table; idxs[q, k] = table[n1, k] for k < K.
+template <typename T>
+__global__ void lookup_gather_kernel(const T* __restrict__ d1, const int64_t* __restrict__ i1,
+                                     const int64_t* __restrict__ table, int64_t table_k, int64_t rows,
+                                     int K, int64_t* __restrict__ idxs) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows * K) return;
+  const int64_t row = i / K;
+  const int k = static_cast<int>(i - row * K);
+  const int64_t n1 = d1[row] < static_cast<T>(1e10) ? i1[row] : 0;
+  idxs[i] = table[n1 * table_k + k];
+}
+
+hipError_t launch_lookup_gather(const void* d1, const int64_t* i1, const int64_t* table, int64_t table_k,
+                                int64_t rows, int64_t K, int32_t dtype, int64_t* idxs, hipStream_t st) {
+  const int64_t n = rows * K;
+  if (n == 0) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+  if (dtype == AKNN_F32)
+    hipLaunchKernelGGL(lookup_gather_kernel<float>, grid, block, 0, st, static_cast<const float*>(d1), i1, table,
+                       table_k, rows, static_cast<int>(K), idxs);
+  else
+    hipLaunchKernelGGL(lookup_gather_kernel<double>, grid, block, 0, st, static_cast<const double*>(d1), i1,
+                       table, table_k, rows, static_cast<int>(K), idxs);
+  return hipGetLastError();
+}
 
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st) {
   return second_order ? launch_q<kBwd2>(L, st) : launch_q<kBwd1>(L, st);

@@ -80,6 +80,10 @@ hipError_t launch_pack_points(const void* p, const void* sigma, const void* R, v
                               int64_t rows, hipStream_t st);
 hipError_t launch_knn(const KnnLaunch& L, hipStream_t st);
 hipError_t launch_aggregate(const QueryLaunch& L, hipStream_t st);
+// isotropic forward: L.sigma = one scale per point row, L.w_out = [rows] weight sums
+hipError_t launch_aggregate_iso(const QueryLaunch& L, hipStream_t st);
+hipError_t launch_lookup_gather(const void* d1, const int64_t* i1, const int64_t* table, int64_t table_k,
+                                int64_t rows, int64_t K, int32_t dtype, int64_t* idxs, hipStream_t st);
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st);
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
 

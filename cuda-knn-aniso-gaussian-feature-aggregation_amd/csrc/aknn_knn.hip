@@ -17,8 +17,8 @@ __global__ void pack_points_kernel(const T* __restrict__ p, const T* __restrict_
   const int c = static_cast<int>(i & 15);
   T v;
   if (c < 3) v = p[n * 3 + c];
-  else if (c < 12) v = R[n * 9 + (c - 3)];
-  else if (c < 15) v = sigma[n * 3 + (c - 12)];
+  else if (c < 12) v = R ? R[n * 9 + (c - 3)] : T(0);           // R / sigma may be absent for the
+  else if (c < 15) v = sigma ? sigma[n * 3 + (c - 12)] : T(0);  // Euclidean key (never read then)
   else v = T(0);
   rec[i] = v;
 }

@@ -272,8 +272,9 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     const int64_t row = perm[i0 + lane];
     T v[16];
     for (int c = 0; c < 3; ++c) v[c] = p[row * 3 + c];
-    for (int c = 0; c < 9; ++c) v[3 + c] = R[row * 9 + c];
-    for (int c = 0; c < 3; ++c) v[12 + c] = sigma[row * 3 + c];
+    // R / sigma may be absent for the Euclidean key, which never reads them
+    for (int c = 0; c < 9; ++c) v[3 + c] = R ? R[row * 9 + c] : T(0);
+    for (int c = 0; c < 3; ++c) v[12 + c] = sigma ? sigma[row * 3 + c] : T(0);
     const int idx = static_cast<int>(row - pb);
     if constexpr (sizeof(T) == 4) v[15] = __int_as_float(idx);
     else v[15] = __longlong_as_double(static_cast<long long>(idx));

@@ -21,6 +21,8 @@ from . import _lib
 __all__ = [
     "knn_aggregate_aniso_forward",
     "knn_precomputed_aggregate_aniso_forward",
+    "knn_lookup_aggregate_aniso_forward",
+    "knn_aggregate_forward",
     "knn_aggregate_aniso_backward",
     "knn_aggregate_aniso_backward_2nd",
     "knn_aniso",
@@ -214,6 +216,95 @@ def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, k
                                       f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
                                       _stream(dev)))
     return [f_out, w_out, dists]
+
+
+def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
+    """Table-lookup forward -> ``[f_out, w_out, dists, idxs]``.
+
+    Reference: ``knn_lookup_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:50-73,
+    csrc/forward.cu:113-183,341-395).  Each query's Euclidean nearest point ``n1`` (lowest
+    index at the minimum; 0 unless that minimum is below 1e10) picks row ``n1`` of
+    ``knn_table`` [P, >=K] (int64, shared by the batches); its first K entries are the
+    neighbours, in table order, weighted with the anisotropic Gaussians.  The table is
+    range-checked here (one device reduction) because the kernels trust it.
+    """
+    for t, n in zip((q, p, f, sigma, R, knn_table), ("q", "p", "f", "sigma", "R", "knn_table")):
+        _check_input(t, n)
+    dt = _check_float((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R"))
+    B, Q, P, F = _check_shapes(q, p, f, sigma, R)
+    if knn_table.dtype != torch.int64 or knn_table.dim() != 2 or int(knn_table.shape[0]) != P:
+        raise RuntimeError(f"knn_table must be int64 [P, >=K] (got {tuple(knn_table.shape)}, {knn_table.dtype})")
+    if knn_table.device != q.device:
+        raise RuntimeError("knn_table must be on the device of q")
+    K = _check_k(K, P)
+    if int(knn_table.shape[1]) < K:
+        raise RuntimeError(f"knn_table has {knn_table.shape[1]} columns but K = {K}")
+    dev = q.device
+    f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
+    w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
+    dists = torch.empty(B, Q, K, dtype=dt, device=dev)
+    idxs = torch.empty(B, Q, K, dtype=torch.int64, device=dev)
+    if B * Q == 0:
+        return [f_out, w_out, dists, idxs]
+    lo, hi = torch.aminmax(knn_table[:, :K])
+    if int(lo) < 0 or int(hi) >= P:
+        raise RuntimeError(f"knn_table entries must lie in [0, {P}) (got [{int(lo)}, {int(hi)}])")
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt)
+    ws_bytes = lib.aknn_lookup_workspace_size(ctypes.byref(dims))
+    if ws_bytes == 0:
+        raise RuntimeError(f"aknn_lookup_workspace_size failed: {lib.aknn_last_error().decode()}")
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_lookup_forward(ctypes.byref(dims), q.data_ptr(), p.data_ptr(), f.data_ptr(),
+                                           sigma.data_ptr(), R.data_ptr(), knn_table.data_ptr(),
+                                           int(knn_table.stride(0)), f_out.data_ptr(), w_out.data_ptr(),
+                                           dists.data_ptr(), idxs.data_ptr(), ws.data_ptr(), ws_bytes,
+                                           _stream(dev)))
+    return [f_out, w_out, dists, idxs]
+
+
+def knn_aggregate_forward(q, p, f, sigma, K, *, tune=None):
+    """Isotropic forward -> ``[f_out, w_out [B,Q,1], dists, idxs]``.
+
+    Reference: ``knn_aggregate_forward`` (csrc/knn_aggregate.cpp:5-23, csrc/forward.cu:5-53,
+    launcher :233-283; exported but without a Python wrapper or backward there).  The K
+    nearest points by Euclidean distance, ``dists`` = squared distances,
+    ``w = exp(-dists * sigma[..., 0])``, ``f_out = sum w f``, ``w_out`` = the weight sum.
+    ``sigma`` is [B, P, C] with C >= 1 (only channel 0 is read, as in the reference).
+    """
+    for t, n in zip((q, p, f, sigma), ("q", "p", "f", "sigma")):
+        _check_input(t, n)
+    dt = _check_float((q, p, f, sigma), ("q", "p", "f", "sigma"))
+    if q.dim() != 3 or q.shape[2] != 3 or p.dim() != 3 or p.shape[2] != 3:
+        raise RuntimeError("q and p must be [B, Q, 3] and [B, P, 3]")
+    B, Q, P = q.shape[0], q.shape[1], p.shape[1]
+    if p.shape[0] != B or f.dim() != 3 or tuple(f.shape[:2]) != (B, P) or f.shape[2] < 1:
+        raise RuntimeError(f"f must be [B, P, F] with F >= 1 (got {tuple(f.shape)})")
+    if sigma.dim() != 3 or tuple(sigma.shape[:2]) != (B, P) or sigma.shape[2] < 1:
+        raise RuntimeError(f"sigma must be [B, P, C] with C >= 1 (got {tuple(sigma.shape)})")
+    F = f.shape[2]
+    K = _check_k(K, P)
+    dev = q.device
+    scale = sigma[..., 0].contiguous()
+    f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
+    w_out = torch.empty(B, Q, 1, dtype=dt, device=dev)
+    dists = torch.empty(B, Q, K, dtype=dt, device=dev)
+    idxs = torch.empty(B, Q, K, dtype=torch.int64, device=dev)
+    if B * Q == 0:
+        return [f_out, w_out, dists, idxs]
+    lib = _lib.load()
+    dims = make_dims(B, Q, P, F, K, dt, "euclidean")
+    ws_bytes = lib.aknn_iso_workspace_size(ctypes.byref(dims))
+    if ws_bytes == 0:
+        raise RuntimeError(f"aknn_iso_workspace_size failed: {lib.aknn_last_error().decode()}")
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    tn = _tune(tune)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_iso_forward(ctypes.byref(dims), _tp(tn), q.data_ptr(), p.data_ptr(), f.data_ptr(),
+                                        scale.data_ptr(), f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
+                                        idxs.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
+    return [f_out, w_out, dists, idxs]
 
 
 def knn_aniso(q, p, sigma, R, K, *, key: str = "aniso", tune=None):

@@ -19,7 +19,9 @@ from . import _C, _lib
 
 __all__ = [
     "knn_aggregate_aniso",
+    "knn_aggregate_aniso_lookup",
     "KNNAnisotropicAggregate",
+    "KNNLookupAnisotropicAggregate",
     "KNNPrecomputedAnisotropicAggregate",
     "KNNAnisotropicAggregateBackward",
 ]
@@ -54,6 +56,34 @@ def knn_aggregate_aniso(
         idxs: [B, Q, K] k-NN indices (int64). Not differentiable.
     """
     return KNNAnisotropicAggregate.apply(q, p, f, sigma, R, K, key)
+
+
+def knn_aggregate_aniso_lookup(
+    q: torch.Tensor,
+    p: torch.Tensor,
+    f: torch.Tensor,
+    sigma: torch.Tensor,
+    R: torch.Tensor,
+    knn_table: torch.Tensor,
+    K: int,
+):
+    """k-NN feature aggregate using anisotropic gaussian weights, with the neighbours
+    taken from a pre-computed table (reference: cuda_knn_aggregate/__init__.py:34-62).
+
+    Args:
+        q, p, f, sigma, R: as in :func:`knn_aggregate_aniso`.
+        knn_table: [P, >=K] int64 table of neighbour indices per surface point (for
+            example the kNN of p among itself); shared by the B batches.
+        K: number of table entries used per query.
+
+    Each query's Euclidean nearest surface point selects its table row; the first K
+    entries of that row are the neighbours (in table order).  Gradients as in
+    :func:`knn_aggregate_aniso`, given those neighbours.
+
+    Returns:
+        f_out [B, Q, F], w_out [B, Q, K], dists [B, Q, K], idxs [B, Q, K] (int64).
+    """
+    return KNNLookupAnisotropicAggregate.apply(q, p, f, sigma, R, knn_table, K)
 
 
 def _point_needs(flags):
@@ -107,6 +137,25 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
             cache,
         )
         return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None
+
+
+class KNNLookupAnisotropicAggregate(KNNAnisotropicAggregate):
+    """Table-lookup neighbours + aggregation (reference: cuda_knn_aggregate/__init__.py:120-165).
+    ``apply(q, p, f, sigma, R, knn_table, K) -> (f_out, w_out, dists, idxs)``."""
+
+    @staticmethod
+    def forward(ctx, q, p, f, sigma, R, knn_table, K):
+        f_out, w_out, dists, idxs = _C.knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K)
+        ctx.save_for_backward(q, p, f, sigma, R, w_out, idxs)
+        ctx.key = "aniso"
+        ctx.has_cache = False
+        ctx.set_materialize_grads(False)
+        return f_out, w_out, dists, idxs
+
+    @staticmethod
+    def backward(ctx, grad_f_out, grad_w_out, grad_dists, grad_idxs):
+        grads = KNNAnisotropicAggregate.backward(ctx, grad_f_out, grad_w_out, grad_dists, grad_idxs)
+        return grads[:5] + (None, None)
 
 
 class KNNPrecomputedAnisotropicAggregate(KNNAnisotropicAggregate):

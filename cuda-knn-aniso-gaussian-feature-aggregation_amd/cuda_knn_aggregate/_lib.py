@@ -50,6 +50,10 @@ EXPORTS = (
     "aknn_backward_workspace_size",
     "aknn_backward",
     "aknn_backward2",
+    "aknn_lookup_workspace_size",
+    "aknn_lookup_forward",
+    "aknn_iso_workspace_size",
+    "aknn_iso_forward",
 )
 
 
@@ -111,6 +115,11 @@ _SIGS = {
                       + [_P] * 5 + [_P]),
     "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE, _P, ctypes.c_size_t]
                        + [_P] * 6 + [_P]),
+    "aknn_lookup_workspace_size": (ctypes.c_size_t, [_DIMS]),
+    "aknn_lookup_forward": (ctypes.c_int, [_DIMS] + [_P] * 6 + [ctypes.c_int64] + [_P] * 4
+                            + [_P, ctypes.c_size_t, _P]),
+    "aknn_iso_workspace_size": (ctypes.c_size_t, [_DIMS]),
+    "aknn_iso_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 4 + [_P] * 4 + [_P, ctypes.c_size_t, _P]),
 }
 
 

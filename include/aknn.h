@@ -25,6 +25,9 @@
  *                      launcher csrc/backward.cu:331-389
  *   aknn_backward2  <- knn_aggregate_aniso_backward_2nd   csrc/knn_aggregate.cpp:135-172,
  *                      launcher csrc/backward.cu:392-455
+ *   aknn_aggregate  <- knn_precomputed_aggregate_aniso_forward  csrc/knn_aggregate.cpp:76-99
+ *   aknn_lookup_forward <- knn_lookup_aggregate_aniso_forward  csrc/knn_aggregate.cpp:50-73
+ *   aknn_iso_forward    <- knn_aggregate_forward               csrc/knn_aggregate.cpp:5-23
  *   (pybind11 table csrc/knn_aggregate.cpp:178-185)
  */
 #ifndef AKNN_H_
@@ -155,6 +158,32 @@ AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* 
 AKNN_API int aknn_aggregate(const aknn_dims* dims, const void* q, const void* p, const void* f,
                             const void* sigma, const void* R, const int64_t* idxs, void* f_out,
                             void* w_out, void* dists, void* stream);
+
+/* Table-lookup forward.  Replaces knn_lookup_aggregate_aniso_forward
+ * (csrc/knn_aggregate.cpp:50-73; kernel csrc/forward.cu:113-183, launcher :341-395).  For
+ * each query: n1 = its Euclidean nearest point (the lowest index reaching the minimum of
+ * x^2+y^2+z^2, and 0 unless that minimum is below 1e10, as the reference's scan
+ * started from 1e10 gives); neighbours idxs[q, k] = knn_table[n1, k] for k < K (table
+ * order, not sorted); then dists = anisotropic D, w_out = exp(-D) and f_out as in
+ * aknn_aggregate.  knn_table: int64 [P, table_k] shared by all B batches, entries in
+ * [0, P) (not checked here); K <= table_k.  Dense batches only.  The backward passes
+ * take its idxs like any other. */
+AKNN_API size_t aknn_lookup_workspace_size(const aknn_dims* dims);
+AKNN_API int aknn_lookup_forward(const aknn_dims* dims, const void* q, const void* p, const void* f,
+                                 const void* sigma, const void* R, const int64_t* knn_table,
+                                 int64_t table_k, void* f_out, void* w_out, void* dists, int64_t* idxs,
+                                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* Isotropic forward.  Replaces knn_aggregate_forward (csrc/knn_aggregate.cpp:5-23;
+ * kernel csrc/forward.cu:5-53, launcher :233-283): the K nearest points by Euclidean
+ * distance ((key, index) order), dists = x^2+y^2+z^2, w_k = exp(-(dists_k * scale[n_k])),
+ * f_out = sum_k w_k f[n_k], w_out[B, Q] (one value per query, the reference's [B, Q, 1]) =
+ * sum_k w_k.  scale: one value per point row (the reference reads sigma[b, n, 0]).
+ * dims->key is ignored.  No backward (the reference has none). */
+AKNN_API size_t aknn_iso_workspace_size(const aknn_dims* dims);
+AKNN_API int aknn_iso_forward(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
+                              const void* f, const void* scale, void* f_out, void* w_out, void* dists,
+                              int64_t* idxs, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Point-side transpose of idxs (CSR over the B*P point rows, each row's entries
  * q*K+k in ascending order), needed by the point gradients of both backward

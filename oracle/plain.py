@@ -13,6 +13,8 @@ Reference anchors (paths relative to the reference repository):
 * second-order grads:  csrc/backward.cu:152-327 (only ``grad_grad_q`` used,
                        cuda_knn_aggregate/__init__.py:260-289)
 * zero-initialised outputs: csrc/forward.cu:310-315, csrc/backward.cu:360-364,424-429
+* table lookup:        csrc/forward.cu:113-183 (Euclidean 1-NN, then a [P, K] table row)
+* isotropic forward:   csrc/forward.cu:5-53
 
 Contract choices that differ from the reference (documented, not accidental):
 
@@ -281,6 +283,47 @@ def eikonal_step(q, p, f, sigma, R, K, v, key="aniso", select=knn_select):
     L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
     grads = torch.autograd.grad(L, ins)
     return L, grads
+
+
+# --------------------------------------------------------------------------- lookup
+def nearest_point(q, p):
+    """Euclidean 1-NN per query as the reference's lookup scan finds it
+    (csrc/forward.cu:135-157): running minimum started at 1e10 with a strict ``<`` over
+    ascending point index, so the lowest index at the minimum wins and index 0 is kept
+    when no key is below 1e10 (NaN keys never win).  Returns int64 [B, Q]."""
+    B, Q, _ = q.shape
+    P = p.shape[1]
+    out = torch.empty(B, Q, dtype=torch.int64)
+    thr = torch.tensor(1e10, dtype=q.dtype)
+    step = _chunk_rows(P)
+    for b in range(B):
+        for s in range(0, Q, step):
+            D = euclid_key(q[b, s:s + step], p[b])
+            D = torch.where(D < thr, D, torch.full_like(D, float("inf")))
+            vals, idx = torch.min(D, dim=1)          # first index of the minimum
+            out[b, s:s + step] = torch.where(torch.isinf(vals), torch.zeros_like(idx), idx)
+    return out
+
+
+def lookup_idxs(q, p, knn_table, K):
+    """Neighbours of the table lookup (csrc/forward.cu:160-162): row ``nearest_point``
+    of the [P, >=K] table, first K entries, table order."""
+    return knn_table[nearest_point(q, p)][..., :K]
+
+
+# --------------------------------------------------------------------------- isotropic
+def iso_forward(q, p, f, sigma, K):
+    """The reference's isotropic forward (csrc/forward.cu:5-53): K nearest by Euclidean
+    key in the (key, index) order, ``w = exp(-(d^2 * sigma[..., 0]))``, ``f_out = sum w f``,
+    ``w_out`` [B,Q,1] = sum of the weights.  Keys in the inputs' dtype (FP32 for the
+    neighbours and dists), weights and sums in FP64.  Returns f_out, w_out, dists, idxs."""
+    B, P = q.shape[0], p.shape[1]
+    dists, idxs = knn_sort(q, p, torch.zeros_like(p), torch.zeros(B, P, 3, 3, dtype=p.dtype), K, key="euclidean")
+    s0 = gather_points(sigma[..., 0:1], idxs)[..., 0].double()
+    w = torch.exp(-(dists.double() * s0))
+    fn = gather_points(f.double(), idxs)
+    f_out = (w[..., None] * fn).sum(2)
+    return f_out, w.sum(-1, keepdim=True), dists, idxs
 
 
 def rel_err(a: torch.Tensor, b: torch.Tensor) -> float:

@@ -285,3 +285,79 @@ def test_eikonal_step_runs_and_matches_finite_difference():
         Lm = plain.eikonal_step(b.q, b.p, b.f, b.sigma, b.R, 5, b.v, select=plain.knn_sort)[0].item()
         x[j] = old
         assert abs((Lp - Lm) / (2 * eps) - grads[2].reshape(-1)[j].item()) < 1e-6
+
+
+# ----------------------------------------------------------------------------- lookup
+def _nearest_loop(q, p):
+    """The reference's scan written out literally (csrc/forward.cu:135-157)."""
+    B, Q, P = q.shape[0], q.shape[1], p.shape[1]
+    out = torch.zeros(B, Q, dtype=torch.int64)
+    for b in range(B):
+        for i in range(Q):
+            best, arg = torch.tensor(1e10, dtype=q.dtype), 0
+            for n in range(P):
+                x = q[b, i, 0] - p[b, n, 0]
+                y = q[b, i, 1] - p[b, n, 1]
+                z = q[b, i, 2] - p[b, n, 2]
+                d = (x * x + y * y) + z * z
+                if d < best:
+                    best, arg = d, n
+            out[b, i] = arg
+    return out
+
+
+def test_nearest_point_matches_literal_scan_and_kdtree():
+    from scipy.spatial import cKDTree
+    b = small(B=2, Q=40, P=30, seed=3)
+    q, p = b.q.clone(), b.p.clone()
+    p[0, 20] = p[0, 5]                     # duplicate: lowest index must win
+    q[0, 0] = p[0, 20]                     # exactly on the duplicated point
+    q[1, 3] = torch.tensor([1e6, 0.0, 0.0])  # every key >= 1e10: index 0
+    p[1, 7] = float("nan")                 # a NaN point never wins
+    got = plain.nearest_point(q, p)
+    assert torch.equal(got, _nearest_loop(q, p))
+    assert got[0, 0] == 5 and got[1, 3] == 0
+    for bi in range(2):                    # distinct distances: the textbook 1-NN
+        ok = torch.isfinite(p[bi]).all(-1)
+        tree = cKDTree(p[bi][ok].double().numpy())
+        _, j = tree.query(q[bi].double().numpy(), k=1)
+        ref = torch.nonzero(ok)[:, 0][torch.from_numpy(j)]
+        near = (q[bi] - p[bi][got[bi]]).norm(dim=-1) < 1e4
+        dg = (q[bi].double() - p[bi][got[bi]].double()).norm(dim=-1)
+        dr = (q[bi].double() - p[bi][ref].double()).norm(dim=-1)
+        assert torch.equal(dg[near], dr[near])      # same nearest distance everywhere
+        uniq = near & (got[bi] != 5) & (got[bi] != 20)  # the tree breaks the planted tie arbitrarily
+        assert torch.equal(got[bi][uniq], ref[uniq])
+
+
+def test_lookup_idxs_takes_the_nearest_points_table_row():
+    b = small(B=1, Q=50, P=30, seed=4)
+    table = torch.stack([torch.roll(torch.arange(30), -n) for n in range(30)])  # row n starts at n
+    idxs = plain.lookup_idxs(b.q, b.p, table, 5)
+    n1 = plain.nearest_point(b.q, b.p)
+    assert torch.equal(idxs[..., 0], n1)
+    assert torch.equal(idxs[..., 1], (n1 + 1) % 30)
+
+
+# ----------------------------------------------------------------------------- isotropic
+def test_iso_forward_k1_closed_form():
+    b = small(B=1, Q=20, P=15, seed=5)
+    f_out, w_out, dists, idxs = plain.iso_forward(b.q, b.p, b.f, b.sigma, 1)
+    n = idxs[0, :, 0]
+    d2 = ((b.q[0].double() - b.p[0, n].double()) ** 2).sum(-1)
+    w = torch.exp(-d2 * b.sigma[0, n, 0].double())
+    assert torch.allclose(w_out[0, :, 0], w, rtol=1e-6, atol=0)
+    assert torch.allclose(f_out[0], w[:, None] * b.f[0, n].double(), rtol=1e-6, atol=0)
+
+
+def test_iso_forward_equals_aniso_with_identity_rotation():
+    """R = I and sigma = (s, s, s) turn the anisotropic weight into the isotropic one:
+    D = s (x^2 + y^2 + z^2)."""
+    b = small(B=2, Q=30, P=25, seed=6)
+    s = b.sigma[..., :1].expand(-1, -1, 3).contiguous()
+    R = torch.eye(3).expand(2, 25, 3, 3).contiguous()
+    f_iso, w_iso, d_iso, i_iso = plain.iso_forward(b.q, b.p, b.f, s, 4)
+    _, i_an = plain.knn_sort(b.q, b.p, s, R, 4, key="euclidean")
+    f_an, w_an, _ = plain.forward_given_idxs(*(t.double() for t in (b.q, b.p, b.f, s, R)), i_an)
+    assert torch.equal(i_iso, i_an)
+    assert plain.rel_err(f_iso, f_an) < 1e-6 and plain.rel_err(w_iso[..., 0], w_an.sum(-1)) < 1e-6
