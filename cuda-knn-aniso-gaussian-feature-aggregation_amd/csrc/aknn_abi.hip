@@ -620,10 +620,17 @@ int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets
 
 // dq via per-entry contributions when the point kernels run anyway.
 bool contrib_path(const aknn_dims* d, uint32_t needs) {
-  return (needs & AKNN_NEED_DQ) && (needs & AKNN_NEED_POINTS) && point_rows(d) > 0 && query_rows(d) > 0;
+  return (needs & AKNN_NEED_DQ) && ((needs & AKNN_NEED_POINTS) || (needs & AKNN_SECOND_FULL)) &&
+         point_rows(d) > 0 && query_rows(d) > 0;
 }
 
-static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, const void* g,
+struct Upstream2 {  // the second order's upstream beyond grad_grad_q (full second order)
+  const void *p, *f, *sigma, *R;
+  bool any() const { return p || f || sigma || R; }
+};
+
+static int backward_common(const aknn_dims* d, uint32_t needs, bool second, const void* gg,
+                           const Upstream2& up, const void* g,
                            const void* q, const void* p, const void* f, const void* sigma,
                            const void* R, const int64_t* idxs, const int32_t* csr_off,
                            const int32_t* csr_ent, const aknn_cache* cache, void* ws,
@@ -632,7 +639,10 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
   int rc = check_dims(d, true);
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool second = gg != nullptr;
+  const bool full = second && up.any();
+  if (full && !(needs & AKNN_SECOND_FULL))
+    return fail(AKNN_E_INVALID, "grad_grad_p / _f / _sigma / _R given: set AKNN_SECOND_FULL in needs");
+  if (!full) needs &= ~AKNN_SECOND_FULL;
   const bool via_contrib = contrib_path(d, needs);
   if (via_contrib && (!ws || ws_bytes < aknn_backward_workspace_size(d, needs)))
     return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
@@ -658,6 +668,9 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.idxs = idxs;
     L.grad_f_out = g;
     L.grad_grad_q = gg;
+    L.gg_p = up.p;
+    L.gg_sigma = up.sigma;
+    L.gg_R = up.R;
     L.dq = ((needs & AKNN_NEED_DQ) && !via_contrib) ? dq : nullptr;
     L.d_grad_f_out = (needs & AKNN_NEED_DGRAD_F_OUT) ? dgf : nullptr;
     L.order = cache ? cache->query_order : nullptr;
@@ -665,11 +678,27 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     hipError_t e;
     {
       Span sp(second ? "query_grad2" : "query_grad", st);
-      e = aknn::launch_query_grad(L, second, st);
+      e = full ? aknn::launch_query_grad_full(L, st) : aknn::launch_query_grad(L, second, st);
     }
     if (e != hipSuccess) return hip_fail(e, second ? "query_grad2" : "query_grad");
+    if (full && up.f && (needs & AKNN_NEED_DGRAD_F_OUT)) {
+      // d grad_f_out += sum_k w_k grad_grad_f[n_k]: the forward aggregation of gg_f
+      aknn::QueryLaunch A = L;
+      A.f = up.f;
+      A.dists = nullptr;
+      A.dists_out = nullptr;
+      A.w_out = nullptr;
+      A.f_out = dgf;
+      A.accumulate = true;
+      A.key = AKNN_KEY_ANISO;
+      {
+        Span sp("query_grad2_ggf", st);
+        e = aknn::launch_aggregate(A, st);
+      }
+      if (e != hipSuccess) return hip_fail(e, "query_grad2_ggf");
+    }
   }
-  if ((needs & AKNN_NEED_POINTS) && point_rows(d) > 0) {
+  if (((needs & AKNN_NEED_POINTS) || via_contrib) && point_rows(d) > 0) {
     if (!csr_off || (query_rows(d) > 0 && !csr_ent))
       return fail(AKNN_E_INVALID, "point gradients need the CSR from aknn_build_csr");
     aknn::PointLaunch L{};
@@ -682,7 +711,12 @@ static int backward_common(const aknn_dims* d, uint32_t needs, const void* gg, c
     L.csr_ent = csr_ent;
     L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
     L.grad_f_out = g;
+    L.second = second;
     L.grad_grad_q = gg;
+    L.gg_p = up.p;
+    L.gg_f = up.f;
+    L.gg_sigma = up.sigma;
+    L.gg_R = up.R;
     L.dp = (needs & AKNN_NEED_DP) ? dp : nullptr;
     L.df = (needs & AKNN_NEED_DF) ? df : nullptr;
     L.dsigma = (needs & AKNN_NEED_DSIGMA) ? dsigma : nullptr;
@@ -717,7 +751,8 @@ int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, co
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
                   const aknn_cache* cache, void* ws, size_t ws_bytes, void* dq, void* dp, void* df,
                   void* dsigma, void* dR, void* stream) {
-  return backward_common(d, needs & ~AKNN_NEED_DGRAD_F_OUT, nullptr, grad_f_out, q, p, f, sigma, R,
+  return backward_common(d, needs & ~(AKNN_NEED_DGRAD_F_OUT | AKNN_SECOND_FULL), false, nullptr, Upstream2{},
+                         grad_f_out, q, p, f, sigma, R,
                          idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, nullptr, dq, dp, df,
                          dsigma, dR, stream);
 }
@@ -729,9 +764,23 @@ int aknn_backward2(const aknn_dims* d, uint32_t needs, const void* grad_grad_q,
                    const aknn_cache* cache, void* ws, size_t ws_bytes, void* d_grad_f_out,
                    void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
   if (!grad_grad_q) return fail(AKNN_E_INVALID, "grad_grad_q is NULL");
-  return backward_common(d, needs, grad_grad_q, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets,
-                         csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp, df, dsigma, dR,
-                         stream);
+  return backward_common(d, needs & ~AKNN_SECOND_FULL, true, grad_grad_q, Upstream2{}, grad_f_out, q, p, f,
+                         sigma, R, idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp,
+                         df, dsigma, dR, stream);
+}
+
+int aknn_backward2_full(const aknn_dims* d, uint32_t needs, const void* grad_grad_q, const void* grad_grad_p,
+                        const void* grad_grad_f, const void* grad_grad_sigma, const void* grad_grad_R,
+                        const void* grad_f_out, const void* q, const void* p, const void* f,
+                        const void* sigma, const void* R, const int64_t* idxs,
+                        const int32_t* csr_offsets, const int32_t* csr_entries,
+                        const aknn_cache* cache, void* ws, size_t ws_bytes, void* d_grad_f_out,
+                        void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
+  const Upstream2 up{grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R};
+  if (!grad_grad_q && !up.any()) return fail(AKNN_E_INVALID, "every upstream gradient is NULL");
+  return backward_common(d, needs | AKNN_SECOND_FULL, true, grad_grad_q, up, grad_f_out, q, p, f, sigma, R,
+                         idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp, df, dsigma,
+                         dR, stream);
 }
 
 }  // extern "C"

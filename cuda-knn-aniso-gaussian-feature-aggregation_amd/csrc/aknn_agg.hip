@@ -59,7 +59,9 @@ __device__ __forceinline__ double row_get(double v) {
 
 // kIso: the reference's isotropic forward (csrc/forward.cu:5-53): Euclidean keys,
 // w = exp(-(d^2 * s_n)) with one scale per point, w_out = sum of the K weights.
-enum { kFwd = 0, kBwd1 = 1, kBwd2 = 2, kIso = 3 };
+// kBwd2F: d grad_f_out of the full second order (upstream also on dp, dsigma, dR):
+// sum_k w Phi f[n_k]; dq then comes from the point kernels' per-entry shares.
+enum { kFwd = 0, kBwd1 = 1, kBwd2 = 2, kIso = 3, kBwd2F = 4 };
 
 // Unrolled loop over the 16 slots of a batch (J must be a compile-time constant for
 // the swizzle pattern).
@@ -124,11 +126,13 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     }
   }
   T hx = T(0), hy = T(0), hz = T(0);
-  if constexpr (MODE == kBwd2) {
+  if constexpr (MODE == kBwd2 || MODE == kBwd2F) {
     const T* __restrict__ gg = static_cast<const T*>(L.grad_grad_q);
-    hx = gg[row * 3 + 0];
-    hy = gg[row * 3 + 1];
-    hz = gg[row * 3 + 2];
+    if (gg) {
+      hx = gg[row * 3 + 0];
+      hy = gg[row * 3 + 1];
+      hz = gg[row * 3 + 2];
+    }
   }
   T dqx = T(0), dqy = T(0), dqz = T(0);
   const bool want_dq = L.dq != nullptr;  // uniform: otherwise only d grad_f_out is made
@@ -165,12 +169,19 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
           D = pair_key<T, AKNN_KEY_ANISO>(qx, qy, qz, pt.p0, pt.p1, pt.p2, pt.r[0], pt.r[1], pt.r[2],
                                           pt.r[3], pt.r[4], pt.r[5], pt.r[6], pt.r[7], pt.r[8], pt.s0,
                                           pt.s1, pt.s2);
-          if (row_ok) static_cast<T*>(L.dists_out)[row * K + k] = D;
+          if (row_ok && L.dists_out) static_cast<T*>(L.dists_out)[row * K + k] = D;
         }
         w = gexp(-D);
-        if (row_ok) static_cast<T*>(L.w_out)[row * K + k] = w;
+        if (row_ok && L.w_out) static_cast<T*>(L.w_out)[row * K + k] = w;
       }
       coef = w;
+    } else if constexpr (MODE == kBwd2F) {
+      pt = fetch_point<T>(rec, p, sg, R, pb + n);
+      geo = geometry<T>(qx, qy, qz, pt);
+      w = have ? geo.w : T(0);
+      const PtUp<T> up = load_up<T>(static_cast<const T*>(L.gg_p), static_cast<const T*>(L.gg_sigma),
+                                    static_cast<const T*>(L.gg_R), pb + n);
+      coef = w * second_phi<T>(geo, pt, hx, hy, hz, up).phi;  // d grad_f_out = sum_k w Phi f[n]
     } else {
       pt = fetch_point<T>(rec, p, sg, R, pb + n);
       geo = geometry<T>(qx, qy, qz, pt);
@@ -251,12 +262,22 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     if (row_ok && r == 0) static_cast<T*>(L.w_out)[row] = wsum;
   }
   if constexpr (MODE != kBwd1) {
-    T* __restrict__ out = static_cast<T*>(MODE == kBwd2 ? L.d_grad_f_out : L.f_out);
+    T* __restrict__ out = static_cast<T*>((MODE == kBwd2 || MODE == kBwd2F) ? L.d_grad_f_out : L.f_out);
     if (row_ok && out) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
-        if (ft < F) vstore<T, VEC>(out + row * F + ft, acc[c]);
+        if (ft < F) {
+          if (MODE == kFwd && L.accumulate) {  // add into the existing row
+            T cur[VEC];
+            vload<T, VEC>(cur, out + row * F + ft);
+#pragma unroll
+            for (int i = 0; i < VEC; ++i) cur[i] += acc[c][i];
+            vstore<T, VEC>(out + row * F + ft, cur);
+          } else {
+            vstore<T, VEC>(out + row * F + ft, acc[c]);
+          }
+        }
       }
     }
   }
@@ -348,6 +369,8 @@ hipError_t launch_lookup_gather(const void* d1, const int64_t* i1, const int64_t
                        table, table_k, rows, static_cast<int>(K), idxs);
   return hipGetLastError();
 }
+
+hipError_t launch_query_grad_full(const QueryLaunch& L, hipStream_t st) { return launch_q<kBwd2F>(L, st); }
 
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st) {
   return second_order ? launch_q<kBwd2>(L, st) : launch_q<kBwd1>(L, st);

@@ -197,6 +197,62 @@ __device__ __forceinline__ Geo<T> geometry(T qx, T qy, T qz, const Pt<T>& P) {
   return g;
 }
 
+// ------------------------------------------------------------------ full second order
+// Upstream gradients of one point's first-order outputs (Hp = d/d dp, Hs = d/d dsigma,
+// HR = d/d dR, row-major); zero when absent.
+template <typename T>
+struct PtUp {
+  T p[3], s[3], r[9];
+};
+
+template <typename T>
+__device__ __forceinline__ PtUp<T> load_up(const T* __restrict__ gp, const T* __restrict__ gs,
+                                           const T* __restrict__ gr, int64_t row) {
+  PtUp<T> u;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    u.p[i] = gp ? gp[row * 3 + i] : T(0);
+    u.s[i] = gs ? gs[row * 3 + i] : T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) u.r[i] = gr ? gr[row * 9 + i] : T(0);
+  return u;
+}
+
+// Per (query, point) entry of the second order for upstream (h = grad_grad_q[q], Hp,
+// Hs, HR): the entry adds  c_e = a w Phi + w (g . Hf)  to <upstream, first-order
+// outputs>, with hh = h - Hp, s = v . hh and
+//   Phi = -2 s - sum_i Hs_i u_i^2 - 2 sum_i sig_i u_i (HR_i . d).
+// With only h this is the reference's -2 s (csrc/backward.cu:196-209).
+template <typename T>
+struct Second2 {
+  T hh[3];   // h - Hp
+  T rh[3];   // R_i . hh
+  T hrd[3];  // HR_i . d
+  T phi;
+};
+
+template <typename T>
+__device__ __forceinline__ Second2<T> second_phi(const Geo<T>& g, const Pt<T>& P, T hx, T hy, T hz,
+                                                 const PtUp<T>& up) {
+  Second2<T> o;
+  o.hh[0] = hx - up.p[0];
+  o.hh[1] = hy - up.p[1];
+  o.hh[2] = hz - up.p[2];
+  const T u[3] = {g.u0, g.u1, g.u2};
+  const T dd[3] = {g.d0, g.d1, g.d2};
+  const T sg[3] = {P.s0, P.s1, P.s2};
+  T phi = T(-2) * (o.hh[0] * g.v0 + o.hh[1] * g.v1 + o.hh[2] * g.v2);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    o.rh[i] = P.r[3 * i] * o.hh[0] + P.r[3 * i + 1] * o.hh[1] + P.r[3 * i + 2] * o.hh[2];
+    o.hrd[i] = up.r[3 * i] * dd[0] + up.r[3 * i + 1] * dd[1] + up.r[3 * i + 2] * dd[2];
+    phi -= up.s[i] * u[i] * u[i] + T(2) * sg[i] * u[i] * o.hrd[i];
+  }
+  o.phi = phi;
+  return o;
+}
+
 // ------------------------------------------------------------------ cross-lane
 // DPP moves inside a row of 16 lanes (gfx9 encodings).
 constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]

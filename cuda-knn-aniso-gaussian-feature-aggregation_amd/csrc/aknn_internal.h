@@ -52,7 +52,9 @@ struct QueryLaunch {
   const void* dists;       // forward: keys from the top-K (nullptr: recompute from idxs)
   void* dists_out;         // forward with recomputed keys: where to store them
   const void* grad_f_out;  // backward
-  const void* grad_grad_q; // second order
+  const void* grad_grad_q; // second order (may be null in the full second order)
+  const void *gg_p, *gg_sigma, *gg_R;  // full second order: upstream of dp, dsigma, dR
+  bool accumulate;         // forward: add into f_out instead of overwriting it
   void *f_out, *w_out;     // forward
   void* dq;                // backward (first or second order)
   void* d_grad_f_out;      // second order
@@ -69,7 +71,9 @@ struct PointLaunch {
   const int32_t* csr_ent;
   const void *q, *p, *f, *sigma, *R;
   const void* grad_f_out;
-  const void* grad_grad_q;  // nullptr for first order
+  bool second;              // second order (grad_grad_q and the gg_* may each be null)
+  const void* grad_grad_q;
+  const void *gg_p, *gg_f, *gg_sigma, *gg_R;  // full second order: upstream of dp, df, dsigma, dR
   void *dp, *df, *dsigma, *dR;
   const int32_t* order;     // optional processing order of the point rows (spatial)
   void* dq_contrib;         // optional [query rows * K * 3]: each entry's share of dq
@@ -85,6 +89,8 @@ hipError_t launch_aggregate_iso(const QueryLaunch& L, hipStream_t st);
 hipError_t launch_lookup_gather(const void* d1, const int64_t* i1, const int64_t* table, int64_t table_k,
                                 int64_t rows, int64_t K, int32_t dtype, int64_t* idxs, hipStream_t st);
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st);
+// full second order: d grad_f_out = sum_k w Phi f[n] (no dq)
+hipError_t launch_query_grad_full(const QueryLaunch& L, hipStream_t st);
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
 
 hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st);

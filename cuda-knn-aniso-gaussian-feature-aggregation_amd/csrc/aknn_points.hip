@@ -62,12 +62,15 @@ __device__ __forceinline__ T group_sum(T v) {
 // G = 16-lane groups per point row (1: four rows per wave; 4: one row per wave, its
 // entries dealt to the groups in turn).  Fewer rows in flight keep the gathered query
 // rows of the rows being processed within L2.
-template <typename T, int VEC, int CH, int KEY, bool SECOND, int G>
+// FULL (second order only): the upstream also holds gradients of dp, df, dsigma, dR
+// (L.gg_p / gg_f / gg_sigma / gg_R, each optional); without it only grad_grad_q, as in
+// the reference.
+template <typename T, int VEC, int CH, int KEY, bool SECOND, int G, bool FULL>
 __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const T* __restrict__ q = static_cast<const T*>(L.q);
   const T* __restrict__ f = static_cast<const T*>(L.f);
   const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
-  const T* __restrict__ hh = static_cast<const T*>(L.grad_grad_q);
+  const T* __restrict__ hh = static_cast<const T*>(L.grad_grad_q);  // may be null with FULL
   const int32_t* __restrict__ off = L.csr_off;
   const int32_t* __restrict__ ent = L.csr_ent;
   const int64_t F = L.F;
@@ -110,6 +113,25 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   T acc[15];  // dp[3], dsigma[3], dR[9]
 #pragma unroll
   for (int i = 0; i < 15; ++i) acc[i] = T(0);
+  // full second order: this point's upstream (dp, dsigma, dR parts) and gg_f slice
+  PtUp<T> up{};
+  T hfs[FULL ? CH : 1][VEC];
+  const bool has_hf = FULL && L.gg_f != nullptr;  // uniform
+  if constexpr (FULL) {
+    up = load_up<T>(static_cast<const T*>(L.gg_p), static_cast<const T*>(L.gg_sigma),
+                    static_cast<const T*>(L.gg_R), row);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) hfs[c][i] = T(0);
+      if (has_hf && fok[c]) {
+        const PVec<T, VEC> x =
+            *reinterpret_cast<const PVec<T, VEC>*>(static_cast<const T*>(L.gg_f) + row * F + fofs[c]);
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) hfs[c][i] = x.v[i];
+      }
+    }
+  }
 
   for (int eb = e0; eb < e1; eb += 16 * G) {
     const int e = eb + 16 * gq + r;
@@ -120,12 +142,20 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
     T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
+    Second2<T> sec{};
     if constexpr (SECOND) {
-      hx = hh[qrow * 3 + 0];
-      hy = hh[qrow * 3 + 1];
-      hz = hh[qrow * 3 + 2];
-      s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
-      coef = T(-2) * w * s;  // df += -2 w s g
+      if (hh) {
+        hx = hh[qrow * 3 + 0];
+        hy = hh[qrow * 3 + 1];
+        hz = hh[qrow * 3 + 2];
+      }
+      if constexpr (FULL) {
+        sec = second_phi<T>(geo, pt, hx, hy, hz, up);
+        coef = w * sec.phi;  // df += w Phi g
+      } else {
+        s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+        coef = T(-2) * w * s;  // df += -2 w s g
+      }
     } else {
       coef = w;  // df += w g
     }
@@ -145,9 +175,9 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) xs[J.value][c] = *reinterpret_cast<const PVec<T, VEC>*>(grow + fofs[c]);
     });
-    T part[16];
+    T part[16], part2[FULL ? 16 : 1];
     pstatic_for<0, 16>([&](auto J) {
-      T dot = T(0);
+      T dot = T(0), dot2 = T(0);
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
 #pragma unroll
@@ -155,11 +185,17 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
           const T x = fok[c] ? xs[J.value][c].v[i] : T(0);
           dfa[c][i] += cj[J.value] * x;
           dot += fsl[c][i] * x;
+          if constexpr (FULL) dot2 += hfs[c][i] * x;
         }
       }
       part[J.value] = dot;
+      if constexpr (FULL) part2[J.value] = dot2;
     });
     const T a = row_transpose_sum<T>(part, r);
+    T ghf = T(0);  // g[q] . gg_f[n]
+    if constexpr (FULL) {
+      if (has_hf) ghf = row_transpose_sum<T>(part2, r);
+    }
 
     if (have) {
       const T u[3] = {geo.u0, geo.u1, geo.u2};
@@ -183,6 +219,37 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
           const T cr = T(-2) * aw * sg[i] * u[i];
 #pragma unroll
           for (int j = 0; j < 3; ++j) acc[6 + 3 * i + j] += cr * d[j];
+        }
+      } else if constexpr (FULL) {
+        // c = a Phi + g.Hf;  d c_e/dd = -2 w c v + a w dPhi/dd, with
+        // dPhi/dd = -2 [R^T t + HR^T (sig*u)],  t_i = sig_i (R_i.hh) + Hs_i u_i + sig_i (HR_i.d)
+        const T c = a * sec.phi + ghf;
+        T t[3], su[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          t[i] = sg[i] * sec.rh[i] + up.s[i] * u[i] + sg[i] * sec.hrd[i];
+          su[i] = sg[i] * u[i];
+        }
+        const T vv[3] = {geo.v0, geo.v1, geo.v2};
+        T gd[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const T dphi = T(-2) * (pt.r[j] * t[0] + pt.r[3 + j] * t[1] + pt.r[6 + j] * t[2] +
+                                  up.r[j] * su[0] + up.r[3 + j] * su[1] + up.r[6 + j] * su[2]);
+          gd[j] = T(-2) * w * c * vv[j] + aw * dphi;
+          acc[j] -= gd[j];  // dp = -d/dd
+        }
+        if (cq) {  // this entry's share of dq (= +d/dd)
+          cq[cslot * 3 + 0] = gd[0];
+          cq[cslot * 3 + 1] = gd[1];
+          cq[cslot * 3 + 2] = gd[2];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[3 + i] += -w * u[i] * (u[i] * c + T(2) * a * (sec.rh[i] + sec.hrd[i]));
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            acc[6 + 3 * i + j] += T(-2) * w * (su[i] * d[j] * c + a * (su[i] * sec.hh[j] + d[j] * t[i]));
         }
       } else {
         const T h[3] = {hx, hy, hz};
@@ -267,29 +334,37 @@ static int pick_vec_points(const PointLaunch& L) {
 
 constexpr int kPointGroups = 4;  // 16-lane groups per point row
 
-template <typename T, int VEC, int KEY, bool SECOND>
+template <typename T, int VEC, int KEY, bool SECOND, bool FULL>
 static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
   constexpr int G = kPointGroups;
   const int64_t per = 16 * VEC;
   const int64_t ch = (L.F + per - 1) / per;
   const int64_t rows_per_block = 16 / G;
   const dim3 grid(static_cast<unsigned>((L.rows + rows_per_block - 1) / rows_per_block)), block(256);
-  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G>), grid, block, 0, st, L);
-  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G>), grid, block, 0, st, L);
-  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G>), grid, block, 0, st, L);
-  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G>), grid, block, 0, st, L);
+  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
+  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
+  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
+  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
   else return hipErrorNotSupported;
   return hipGetLastError();
 }
 
-template <typename T, int KEY, bool SECOND>
-static hipError_t launch_p_vec(const PointLaunch& L, hipStream_t st) {
+template <typename T, int KEY, bool SECOND, bool FULL>
+static hipError_t launch_p_vf(const PointLaunch& L, hipStream_t st) {
   const int v = pick_vec_points<T>(L);
   if constexpr (sizeof(T) == 4) {
-    if (v == 4) return launch_p_ch<T, 4, KEY, SECOND>(L, st);
+    if (v == 4) return launch_p_ch<T, 4, KEY, SECOND, FULL>(L, st);
   }
-  if (v == 2) return launch_p_ch<T, 2, KEY, SECOND>(L, st);
-  return launch_p_ch<T, 1, KEY, SECOND>(L, st);
+  if (v == 2) return launch_p_ch<T, 2, KEY, SECOND, FULL>(L, st);
+  return launch_p_ch<T, 1, KEY, SECOND, FULL>(L, st);
+}
+
+template <typename T, int KEY, bool SECOND>
+static hipError_t launch_p_vec(const PointLaunch& L, hipStream_t st) {
+  if constexpr (SECOND) {
+    if (L.gg_p || L.gg_f || L.gg_sigma || L.gg_R) return launch_p_vf<T, KEY, true, true>(L, st);
+  }
+  return launch_p_vf<T, KEY, SECOND, false>(L, st);
 }
 
 template <bool SECOND>
@@ -303,7 +378,7 @@ static hipError_t launch_p(const PointLaunch& L, hipStream_t st) {
 
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
   if (L.rows == 0) return hipSuccess;
-  return L.grad_grad_q ? launch_p<true>(L, st) : launch_p<false>(L, st);
+  return L.second ? launch_p<true>(L, st) : launch_p<false>(L, st);
 }
 
 // ------------------------------------------------------------------ dq from contributions

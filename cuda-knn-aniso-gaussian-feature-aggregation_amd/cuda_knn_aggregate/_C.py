@@ -427,19 +427,34 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
 
 
 def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out,
-                                     k_idxs, *, key: str = "aniso", csr=None, needs=None, cache=None):
-    """Second-order gradients of ``<grad_grad_q, dq>`` ->
-    ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
+                                     k_idxs, *, key: str = "aniso", csr=None, needs=None, cache=None,
+                                     grad_grad_p=None, grad_grad_f=None, grad_grad_sigma=None,
+                                     grad_grad_R=None):
+    """Second-order gradients -> ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
 
     Reference: ``knn_aggregate_aniso_backward_2nd`` (csrc/knn_aggregate.cpp:135-172,
-    csrc/backward.cu:392-455).  ``needs`` (6 bools) skips outputs (``None``).
+    csrc/backward.cu:392-455), the gradients of ``<grad_grad_q, dq>``.  The keyword
+    upstreams ``grad_grad_p`` [B,P,3], ``grad_grad_f`` [B,P,F], ``grad_grad_sigma`` [B,P,3]
+    and ``grad_grad_R`` [B,P,3,3] add the terms the reference drops
+    (cuda_knn_aggregate/__init__.py:271-289); ``grad_grad_q`` may then be ``None``.
+    ``needs`` (6 bools) skips outputs (``None``).
     """
-    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, (grad_grad_q, grad_f_out),
-                                    ("grad_grad_q", "grad_f_out"))
+    ups = {"grad_grad_p": (grad_grad_p, 3), "grad_grad_f": (grad_grad_f, None),
+           "grad_grad_sigma": (grad_grad_sigma, 3), "grad_grad_R": (grad_grad_R, (3, 3))}
+    full = any(t is not None for t, _ in ups.values())
+    if grad_grad_q is None and not full:
+        raise RuntimeError("grad_grad_q is required (or one of grad_grad_p / _f / _sigma / _R)")
+    given = [(t, n) for n, (t, _) in [("grad_grad_q", (grad_grad_q, 0))] + list(ups.items()) if t is not None]
+    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, tuple(t for t, _ in given) + (grad_f_out,),
+                                    tuple(n for _, n in given) + ("grad_f_out",))
     if tuple(grad_f_out.shape) != (B, Q, F):
         raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
-    if tuple(grad_grad_q.shape) != (B, Q, 3):
+    if grad_grad_q is not None and tuple(grad_grad_q.shape) != (B, Q, 3):
         raise RuntimeError(f"grad_grad_q must have shape [B, Q, 3] (got {tuple(grad_grad_q.shape)})")
+    for n, (t, tail) in ups.items():
+        want = (B, P) + ((F,) if tail is None else ((tail,) if isinstance(tail, int) else tail))
+        if t is not None and tuple(t.shape) != want:
+            raise RuntimeError(f"{n} must have shape {list(want)} (got {tuple(t.shape)})")
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
     dev = q.device
@@ -451,14 +466,23 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         "dsigma": torch.empty(B, P, 3, dtype=dt, device=dev) if mask & _lib.NEED_DSIGMA else None,
         "dR": torch.empty(B, P, 3, 3, dtype=dt, device=dev) if mask & _lib.NEED_DR else None,
     }
-    if mask & _lib.NEED_POINTS and csr is None:
+    if (mask & _lib.NEED_POINTS or (full and mask & _lib.NEED_DQ)) and csr is None:
         csr = build_csr(k_idxs, P, key)
     lib = _lib.load()
     dims = make_dims(B, Q, P, F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
-    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
+    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask | (_lib.SECOND_FULL if full else 0))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
+    if full:
+        with torch.cuda.device(dev):
+            _lib.check(lib.aknn_backward2_full(
+                ctypes.byref(dims), mask, P_(grad_grad_q), P_(grad_grad_p), P_(grad_grad_f), P_(grad_grad_sigma),
+                P_(grad_grad_R), grad_f_out.data_ptr(), q.data_ptr(), p.data_ptr(), f.data_ptr(),
+                sigma.data_ptr(), R.data_ptr(), k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
+                P_(ws), ws_bytes, P_(outs["dgf"]), P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]),
+                P_(outs["dsigma"]), P_(outs["dR"]), _stream(dev)))
+        return [outs[n] for n in names]
     with torch.cuda.device(dev):
         _lib.check(lib.aknn_backward2(ctypes.byref(dims), mask, grad_grad_q.data_ptr(),
                                       grad_f_out.data_ptr(), q.data_ptr(), p.data_ptr(),

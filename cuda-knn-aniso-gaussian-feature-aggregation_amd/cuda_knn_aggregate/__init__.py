@@ -86,6 +86,13 @@ def knn_aggregate_aniso_lookup(
     return KNNLookupAnisotropicAggregate.apply(q, p, f, sigma, R, knn_table, K)
 
 
+# Second order through KNNAnisotropicAggregateBackward: True honours upstream gradients
+# on every first-order output (dq, dp, df, dsigma, dR); False keeps only the one on dq,
+# as the reference does (cuda_knn_aggregate/__init__.py:271-289).  The Eikonal loss
+# (a function of dq alone) gets the same result either way.
+FULL_SECOND_ORDER = True
+
+
 def _point_needs(flags):
     return any(flags)
 
@@ -180,10 +187,10 @@ class KNNPrecomputedAnisotropicAggregate(KNNAnisotropicAggregate):
 
 class KNNAnisotropicAggregateBackward(torch.autograd.Function):
     """First-order backward as a differentiable op, whose own backward is the
-    Eikonal second order (reference: cuda_knn_aggregate/__init__.py:216-289).
+    second order (reference: cuda_knn_aggregate/__init__.py:216-289).
 
-    Only ``grad_grad_q`` is honoured in the second order, as in the reference
-    (__init__.py:260-289): gradients arriving on dp, df, dsigma, dR are ignored.
+    The reference honours only ``grad_grad_q`` (__init__.py:260-289); here gradients
+    arriving on dp, df, dsigma and dR enter too unless ``FULL_SECOND_ORDER`` is False.
     """
 
     @staticmethod
@@ -217,12 +224,15 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         ni = ctx.needs_input_grad
         # outputs: d grad_f_out, dq, dp, df, dsigma, dR
         needs = (ni[0], ni[2], ni[3], ni[4], ni[5], ni[6])
-        if grad_grad_q is None or not any(needs):
+        ups = (grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R) if FULL_SECOND_ORDER else (None,) * 4
+        ups = tuple(None if u is None else u.contiguous() for u in ups)
+        full = any(u is not None for u in ups)
+        if (grad_grad_q is None and not full) or not any(needs):
             return (None,) * 12
-        if csr is None and _point_needs(needs[2:]):
+        if csr is None and (_point_needs(needs[2:]) or (full and needs[1])):
             csr = _C.build_csr(idxs, p.shape[1], ctx.key)
         d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
-            grad_grad_q.contiguous(),
+            None if grad_grad_q is None else grad_grad_q.contiguous(),
             grad_f_out.contiguous(),
             grad_w_out.contiguous(),
             q,
@@ -236,5 +246,9 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
             csr=csr,
             needs=needs,
             cache=cache,
+            grad_grad_p=ups[0],
+            grad_grad_f=ups[1],
+            grad_grad_sigma=ups[2],
+            grad_grad_R=ups[3],
         )
         return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None, None

@@ -34,6 +34,7 @@ NEED_DF = 4
 NEED_DSIGMA = 8
 NEED_DR = 16
 NEED_DGRAD_F_OUT = 32
+SECOND_FULL = 64
 NEED_POINTS = NEED_DP | NEED_DF | NEED_DSIGMA | NEED_DR
 
 EXPORTS = (
@@ -50,6 +51,7 @@ EXPORTS = (
     "aknn_backward_workspace_size",
     "aknn_backward",
     "aknn_backward2",
+    "aknn_backward2_full",
     "aknn_lookup_workspace_size",
     "aknn_lookup_forward",
     "aknn_iso_workspace_size",
@@ -115,6 +117,8 @@ _SIGS = {
                       + [_P] * 5 + [_P]),
     "aknn_backward2": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE, _P, ctypes.c_size_t]
                        + [_P] * 6 + [_P]),
+    "aknn_backward2_full": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 14 + [_CACHE, _P, ctypes.c_size_t]
+                            + [_P] * 6 + [_P]),
     "aknn_lookup_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_lookup_forward": (ctypes.c_int, [_DIMS] + [_P] * 6 + [ctypes.c_int64] + [_P] * 4
                             + [_P, ctypes.c_size_t, _P]),

@@ -70,6 +70,7 @@ extern "C" {
 #define AKNN_NEED_DSIGMA 8u
 #define AKNN_NEED_DR 16u
 #define AKNN_NEED_DGRAD_F_OUT 32u /* aknn_backward2 only */
+#define AKNN_SECOND_FULL 64u      /* workspace query for aknn_backward2_full (see there) */
 #define AKNN_NEED_POINTS (AKNN_NEED_DP | AKNN_NEED_DF | AKNN_NEED_DSIGMA | AKNN_NEED_DR)
 
 /* Problem description.  Dense batches: q_off = p_off = NULL and B, Q, P give the
@@ -158,6 +159,25 @@ AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* 
 AKNN_API int aknn_aggregate(const aknn_dims* dims, const void* q, const void* p, const void* f,
                             const void* sigma, const void* R, const int64_t* idxs, void* f_out,
                             void* w_out, void* dists, void* stream);
+
+/* Full second order (no counterpart in the reference, which drops these terms at
+ * cuda_knn_aggregate/__init__.py:271-289): the backward of aknn_backward's map
+ * (grad_f_out, q, p, f, sigma, R) -> (dq, dp, df, dsigma, dR) for an upstream on every
+ * output: grad_grad_q [B,Q,3], grad_grad_p [B,P,3], grad_grad_f [B,P,F],
+ * grad_grad_sigma [B,P,3], grad_grad_R [B,P,3,3], each optional (NULL = zero; not all
+ * NULL).  Outputs as aknn_backward2.  With only grad_grad_q it equals aknn_backward2 up
+ * to rounding.  dq always comes from the point kernels, so the workspace is
+ * aknn_backward_workspace_size(dims, needs | AKNN_SECOND_FULL) and the CSR is required
+ * whenever dq or a point output is requested. */
+AKNN_API int aknn_backward2_full(const aknn_dims* dims, uint32_t needs, const void* grad_grad_q,
+                                 const void* grad_grad_p, const void* grad_grad_f,
+                                 const void* grad_grad_sigma, const void* grad_grad_R,
+                                 const void* grad_f_out, const void* q, const void* p, const void* f,
+                                 const void* sigma, const void* R, const int64_t* idxs,
+                                 const int32_t* csr_offsets, const int32_t* csr_entries,
+                                 const aknn_cache* cache, void* workspace, size_t workspace_bytes,
+                                 void* d_grad_f_out, void* dq, void* dp, void* df, void* dsigma,
+                                 void* dR, void* stream);
 
 /* Table-lookup forward.  Replaces knn_lookup_aggregate_aniso_forward
  * (csrc/knn_aggregate.cpp:50-73; kernel csrc/forward.cu:113-183, launcher :341-395).  For

@@ -267,6 +267,19 @@ def backward2_autograd(h, g, q, p, f, sigma, R, idxs):
     return torch.autograd.grad(G, [gg] + ins, allow_unused=True)
 
 
+def backward2_full_autograd(h, hp, hf, hs, hR, g, q, p, f, sigma, R, idxs):
+    """Full second order by double autograd of :func:`plain_aggregate`: the gradients of
+    ``<h,dq> + <hp,dp> + <hf,df> + <hs,dsigma> + <hR,dR>`` with respect to
+    ``(g, q, p, f, sigma, R)`` (any upstream may be ``None``).  The reference keeps only
+    the ``h`` term (cuda_knn_aggregate/__init__.py:271-289)."""
+    ins = [t.detach().requires_grad_(True) for t in (q, p, f, sigma, R)]
+    gg = g.detach().requires_grad_(True)
+    out = plain_aggregate(*ins, idxs)
+    first = torch.autograd.grad(out, ins, gg, create_graph=True)
+    G = sum((d * u).sum() for d, u in zip(first, (h, hp, hf, hs, hR)) if u is not None)
+    return torch.autograd.grad(G, [gg] + ins, allow_unused=True)
+
+
 def eikonal_step(q, p, f, sigma, R, K, v, key="aniso", select=knn_select):
     """One fwd + 1st + 2nd-order step of the Eikonal loss on the plain CPU formulation.
 

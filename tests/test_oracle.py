@@ -361,3 +361,30 @@ def test_iso_forward_equals_aniso_with_identity_rotation():
     f_an, w_an, _ = plain.forward_given_idxs(*(t.double() for t in (b.q, b.p, b.f, s, R)), i_an)
     assert torch.equal(i_iso, i_an)
     assert plain.rel_err(f_iso, f_an) < 1e-6 and plain.rel_err(w_iso[..., 0], w_an.sum(-1)) < 1e-6
+
+
+# ----------------------------------------------------------------------------- full second order
+def test_full_second_order_reduces_to_the_reference_terms():
+    """With an upstream on dq only, the full second order is the reference's."""
+    b = small(B=1, Q=12, P=9, K=4, F=3, seed=7)
+    ins = [t.double() for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    _, idxs = plain.knn_sort(b.q, b.p, b.sigma, b.R, 4)
+    g = torch.randn(1, 12, 3, dtype=torch.float64)
+    h = torch.randn(1, 12, 3, dtype=torch.float64)
+    full = plain.backward2_full_autograd(h, None, None, None, None, g, *ins, idxs)
+    ref = plain.backward2_closed(h, g, *ins, idxs)
+    for x, y in zip(full, ref):
+        assert plain.rel_err(x, y) < 1e-12
+
+
+def test_full_second_order_gradgradcheck():
+    """Second derivatives of the whole map (q, p, f, sigma, R) -> f_out, K = P (no
+    neighbour changes under perturbation), checked by finite differences."""
+    b = small(B=1, Q=4, P=3, K=3, F=2, seed=8)
+    idxs = torch.arange(3).expand(1, 4, 3).contiguous()
+    ins = [t.double().requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    assert torch.autograd.gradgradcheck(lambda *t: plain.plain_aggregate(*t, idxs), ins, eps=1e-6, atol=1e-5)
+    g = torch.randn(1, 4, 2, dtype=torch.float64)
+    ups = [torch.randn_like(t) for t in ins]
+    got = plain.backward2_full_autograd(*ups, g, *ins, idxs)
+    assert all(x is not None for x in got)
