@@ -677,7 +677,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.rec = cache ? cache->point_records : nullptr;
     hipError_t e;
     {
-      Span sp(second ? "query_grad2" : "query_grad", st);
+      Span sp(full ? "query_grad2_full" : (second ? "query_grad2" : "query_grad"), st);
       e = full ? aknn::launch_query_grad_full(L, st) : aknn::launch_query_grad(L, second, st);
     }
     if (e != hipSuccess) return hip_fail(e, second ? "query_grad2" : "query_grad");
@@ -725,7 +725,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.dq_contrib = via_contrib ? ws : nullptr;
     hipError_t e;
     {
-      Span sp(second ? "point_grad2" : "point_grad", st);
+      Span sp(full ? "point_grad2_full" : (second ? "point_grad2" : "point_grad"), st);
       e = aknn::launch_point_grad(L, st);
     }
     if (e != hipSuccess) return hip_fail(e, second ? "point_grad2" : "point_grad");

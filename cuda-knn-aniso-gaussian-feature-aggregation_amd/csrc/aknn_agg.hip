@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   T hx = T(0), hy = T(0), hz = T(0);
   if constexpr (MODE == kBwd2 || MODE == kBwd2F) {
     const T* __restrict__ gg = static_cast<const T*>(L.grad_grad_q);
-    if (gg) {
+    if (MODE == kBwd2 || gg) {  // always given outside the full second order
       hx = gg[row * 3 + 0];
       hy = gg[row * 3 + 1];
       hz = gg[row * 3 + 2];

@@ -53,13 +53,14 @@ struct QueryLaunch {
   void* dists_out;         // forward with recomputed keys: where to store them
   const void* grad_f_out;  // backward
   const void* grad_grad_q; // second order (may be null in the full second order)
-  const void *gg_p, *gg_sigma, *gg_R;  // full second order: upstream of dp, dsigma, dR
-  bool accumulate;         // forward: add into f_out instead of overwriting it
   void *f_out, *w_out;     // forward
   void* dq;                // backward (first or second order)
   void* d_grad_f_out;      // second order
   const int32_t* order;    // optional processing order of the query rows (spatial)
   const void* rec;         // optional packed point records [rows][16]
+  // (kept last: the argument block of the other modes is unchanged)
+  const void *gg_p, *gg_sigma, *gg_R;  // full second order: upstream of dp, dsigma, dR
+  bool accumulate;         // forward: add into f_out instead of overwriting it
 };
 
 // Point-side work over the CSR transpose of idxs.
@@ -71,12 +72,13 @@ struct PointLaunch {
   const int32_t* csr_ent;
   const void *q, *p, *f, *sigma, *R;
   const void* grad_f_out;
-  bool second;              // second order (grad_grad_q and the gg_* may each be null)
-  const void* grad_grad_q;
-  const void *gg_p, *gg_f, *gg_sigma, *gg_R;  // full second order: upstream of dp, df, dsigma, dR
+  const void* grad_grad_q;  // second order (null in the full second order when absent)
   void *dp, *df, *dsigma, *dR;
   const int32_t* order;     // optional processing order of the point rows (spatial)
   void* dq_contrib;         // optional [query rows * K * 3]: each entry's share of dq
+  // (kept last: the argument block of the first- and second-order kernels is unchanged)
+  const void *gg_p, *gg_f, *gg_sigma, *gg_R;  // full second order: upstream of dp, df, dsigma, dR
+  bool second;              // second order (grad_grad_q and the gg_* may each be null)
 };
 
 template <typename T>

@@ -62,11 +62,202 @@ __device__ __forceinline__ T group_sum(T v) {
 // G = 16-lane groups per point row (1: four rows per wave; 4: one row per wave, its
 // entries dealt to the groups in turn).  Fewer rows in flight keep the gathered query
 // rows of the rows being processed within L2.
-// FULL (second order only): the upstream also holds gradients of dp, df, dsigma, dR
-// (L.gg_p / gg_f / gg_sigma / gg_R, each optional); without it only grad_grad_q, as in
-// the reference.
-template <typename T, int VEC, int CH, int KEY, bool SECOND, int G, bool FULL>
+template <typename T, int VEC, int CH, int KEY, bool SECOND, int G>
 __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
+  const T* __restrict__ q = static_cast<const T*>(L.q);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
+  const T* __restrict__ hh = static_cast<const T*>(L.grad_grad_q);
+  const int32_t* __restrict__ off = L.csr_off;
+  const int32_t* __restrict__ ent = L.csr_ent;
+  const int64_t F = L.F;
+  const int K = static_cast<int>(L.K);
+
+  const int r = threadIdx.x & 15;
+  const int gq = (threadIdx.x >> 4) % G;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gi = blk * (16 / G) + (threadIdx.x >> 4) / G;
+  const bool row_ok = gi < L.rows;
+  const int64_t gc = row_ok ? gi : L.rows - 1;
+  // spatial processing order (when given): neighbouring points share queries, so the
+  // gathered grad_f_out rows stay in L2; results do not depend on it
+  const int64_t row = L.order ? static_cast<int64_t>(L.order[gc]) : gc;
+
+  const Pt<T> pt = load_point<T>(static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
+                                 static_cast<const T*>(L.R), row);
+  const int e0 = off[row];
+  const int e1 = row_ok ? off[row + 1] : e0;  // same trip count for the whole row
+
+  T fsl[CH][VEC], dfa[CH][VEC];
+  int64_t fofs[CH];  // this lane's feature offset per chunk, clamped into the row
+  bool fok[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+    fok[c] = ft < F;
+    fofs[c] = fok[c] ? ft : 0;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      dfa[c][i] = T(0);
+      fsl[c][i] = T(0);
+    }
+    if (ft < F) {
+      const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(f + row * F + ft);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) fsl[c][i] = x.v[i];
+    }
+  }
+  T acc[15];  // dp[3], dsigma[3], dR[9]
+#pragma unroll
+  for (int i = 0; i < 15; ++i) acc[i] = T(0);
+
+  for (int eb = e0; eb < e1; eb += 16 * G) {
+    const int e = eb + 16 * gq + r;
+    const bool have = e < e1;
+    const int entry = have ? ent[e] : 0;
+    const int64_t qrow = have ? static_cast<int64_t>(entry / K) : 0;
+    const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
+    const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
+    const T w = have ? geo.w : T(0);
+    T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
+    if constexpr (SECOND) {
+      hx = hh[qrow * 3 + 0];
+      hy = hh[qrow * 3 + 1];
+      hz = hh[qrow * 3 + 2];
+      s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+      coef = T(-2) * w * s;  // df += -2 w s g
+    } else {
+      coef = w;  // df += w g
+    }
+    const int qi32 = static_cast<int>(qrow);
+
+    // All 16 row gathers of the batch are issued back to back: no per-lane branch in
+    // this loop (lanes past F read a clamped in-row address and mask the value).
+    int qj[16];
+    T cj[16];
+    pstatic_for<0, 16>([&](auto J) {
+      qj[J.value] = prow_get<J.value>(qi32);
+      cj[J.value] = prow_get<J.value>(coef);
+    });
+    PVec<T, VEC> xs[16][CH];
+    pstatic_for<0, 16>([&](auto J) {
+      const T* __restrict__ grow = g + static_cast<int64_t>(qj[J.value]) * F;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) xs[J.value][c] = *reinterpret_cast<const PVec<T, VEC>*>(grow + fofs[c]);
+    });
+    T part[16];
+    pstatic_for<0, 16>([&](auto J) {
+      T dot = T(0);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) {
+          const T x = fok[c] ? xs[J.value][c].v[i] : T(0);
+          dfa[c][i] += cj[J.value] * x;
+          dot += fsl[c][i] * x;
+        }
+      }
+      part[J.value] = dot;
+    });
+    const T a = row_transpose_sum<T>(part, r);
+
+    if (have) {
+      const T u[3] = {geo.u0, geo.u1, geo.u2};
+      const T d[3] = {geo.d0, geo.d1, geo.d2};
+      const T sg[3] = {pt.s0, pt.s1, pt.s2};
+      const T aw = a * w;
+      T* __restrict__ cq = static_cast<T*>(L.dq_contrib);
+      const int64_t cslot = static_cast<int64_t>(entry);
+      if constexpr (!SECOND) {
+        acc[0] += T(2) * aw * geo.v0;
+        acc[1] += T(2) * aw * geo.v1;
+        acc[2] += T(2) * aw * geo.v2;
+        if (cq) {  // this entry's share of dq (= minus its share of dp)
+          cq[cslot * 3 + 0] = T(-2) * aw * geo.v0;
+          cq[cslot * 3 + 1] = T(-2) * aw * geo.v1;
+          cq[cslot * 3 + 2] = T(-2) * aw * geo.v2;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[3 + i] += -aw * u[i] * u[i];
+          const T cr = T(-2) * aw * sg[i] * u[i];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) acc[6 + 3 * i + j] += cr * d[j];
+        }
+      } else {
+        const T h[3] = {hx, hy, hz};
+        T hr[3];  // h . R_i
+#pragma unroll
+        for (int i = 0; i < 3; ++i) hr[i] = pt.r[3 * i] * hx + pt.r[3 * i + 1] * hy + pt.r[3 * i + 2] * hz;
+        const T m0 = pt.s0 * hr[0], m1 = pt.s1 * hr[1], m2 = pt.s2 * hr[2];
+        const T mh0 = pt.r[0] * m0 + pt.r[3] * m1 + pt.r[6] * m2;
+        const T mh1 = pt.r[1] * m0 + pt.r[4] * m1 + pt.r[7] * m2;
+        const T mh2 = pt.r[2] * m0 + pt.r[5] * m1 + pt.r[8] * m2;
+        const T aw2 = T(-2) * aw;
+        const T t0 = T(2) * s * geo.v0 - mh0, t1 = T(2) * s * geo.v1 - mh1, t2 = T(2) * s * geo.v2 - mh2;
+        acc[0] += aw2 * t0;
+        acc[1] += aw2 * t1;
+        acc[2] += aw2 * t2;
+        if (cq) {  // this entry's share of dq: 2 a w (2 s v - M h)
+          cq[cslot * 3 + 0] = T(2) * aw * t0;
+          cq[cslot * 3 + 1] = T(2) * aw * t1;
+          cq[cslot * 3 + 2] = T(2) * aw * t2;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          acc[3 + i] += T(2) * aw * u[i] * (u[i] * s - hr[i]);
+          const T cr = aw2 * sg[i];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            acc[6 + 3 * i + j] += cr * (h[j] * u[i] + hr[i] * d[j] - T(2) * s * u[i] * d[j]);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < 15; ++i) acc[i] = group_sum<G>(row_sum(acc[i]));
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) dfa[c][i] = group_sum<G>(dfa[c][i]);
+  }
+  if (!row_ok || gq != 0) return;
+  T* dp = static_cast<T*>(L.dp);
+  T* ds = static_cast<T*>(L.dsigma);
+  T* dR = static_cast<T*>(L.dR);
+  T* df = static_cast<T*>(L.df);
+  // lane r < 15 stores accumulator r
+  T mine = acc[0];
+#pragma unroll
+  for (int i = 1; i < 15; ++i) mine = (r == i) ? acc[i] : mine;
+  if (r < 3) {
+    if (dp) dp[row * 3 + r] = mine;
+  } else if (r < 6) {
+    if (ds) ds[row * 3 + (r - 3)] = mine;
+  } else if (r < 15) {
+    if (dR) dR[row * 9 + (r - 6)] = mine;
+  }
+  if (df) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+      if (ft < F) {
+        PVec<T, VEC> x;
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) x.v[i] = dfa[c][i];
+        *reinterpret_cast<PVec<T, VEC>*>(df + row * F + ft) = x;
+      }
+    }
+  }
+}
+
+// The full second order as a kernel of its own (the reference-order kernel above stays
+// as it is): the upstream also holds gradients of dp, df, dsigma, dR (L.gg_p / gg_f /
+// gg_sigma / gg_R, each optional).  Instantiated with SECOND = FULL = true.
+template <typename T, int VEC, int CH, int KEY, bool SECOND, int G, bool FULL>
+__global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
   const T* __restrict__ q = static_cast<const T*>(L.q);
   const T* __restrict__ f = static_cast<const T*>(L.f);
   const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
@@ -144,7 +335,9 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
     Second2<T> sec{};
     if constexpr (SECOND) {
-      if (hh) {
+      // grad_grad_q is always given outside the full second order: loaded with no
+      // branch there, so these loads issue together with the query's
+      if (!FULL || hh) {
         hx = hh[qrow * 3 + 0];
         hy = hh[qrow * 3 + 1];
         hz = hh[qrow * 3 + 2];
@@ -341,11 +534,19 @@ static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
   const int64_t ch = (L.F + per - 1) / per;
   const int64_t rows_per_block = 16 / G;
   const dim3 grid(static_cast<unsigned>((L.rows + rows_per_block - 1) / rows_per_block)), block(256);
-  if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
-  else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
-  else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
-  else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G, FULL>), grid, block, 0, st, L);
-  else return hipErrorNotSupported;
+  if constexpr (FULL) {
+    if (ch <= 1) hipLaunchKernelGGL((point_kernel_full<T, VEC, 1, KEY, true, G, true>), grid, block, 0, st, L);
+    else if (ch <= 2) hipLaunchKernelGGL((point_kernel_full<T, VEC, 2, KEY, true, G, true>), grid, block, 0, st, L);
+    else if (ch <= 4) hipLaunchKernelGGL((point_kernel_full<T, VEC, 4, KEY, true, G, true>), grid, block, 0, st, L);
+    else if (ch <= 8) hipLaunchKernelGGL((point_kernel_full<T, VEC, 8, KEY, true, G, true>), grid, block, 0, st, L);
+    else return hipErrorNotSupported;
+  } else {
+    if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G>), grid, block, 0, st, L);
+    else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G>), grid, block, 0, st, L);
+    else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G>), grid, block, 0, st, L);
+    else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G>), grid, block, 0, st, L);
+    else return hipErrorNotSupported;
+  }
   return hipGetLastError();
 }
 

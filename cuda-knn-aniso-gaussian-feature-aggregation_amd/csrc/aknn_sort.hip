@@ -8,7 +8,8 @@
 // Per pass over digit (key >> shift) & 255, with tiles of kTileKeys keys per block:
 //   1. digit_hist: per-tile digit counts (LDS), stored digit-major: counts[d * tiles + t]
 //   2. exclusive scan of counts (3 kernels): start of each (digit, tile) in the output
-//   3. digit_scatter: each tile walks its keys in order, 256 at a time; a key's rank
+//   3. digit_scatter: local digit starts from the tile's own counts (kept by step 1);
+//      each tile walks its keys in order, 256 at a time; a key's rank
 //      among equal digits is (earlier chunks) + (earlier waves of this chunk) +
 //      (earlier lanes of its wave, from 8 ballots over the digit bits).  Keys are
 //      placed in LDS at their tile-local rank, then written out run by run.
@@ -25,7 +26,8 @@ constexpr int kScanChunk = 4096;                      // elements per scan block
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __restrict__ keys, int64_t n,
                                                                   int shift, int64_t tiles,
-                                                                  uint32_t* __restrict__ counts) {
+                                                                  uint32_t* __restrict__ counts,
+                                                                  uint32_t* __restrict__ tile_counts) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -36,7 +38,8 @@ __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __res
     if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);  // integer: order-independent
   }
   __syncthreads();
-  counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+  counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];       // scanned next
+  tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];  // kept per tile
 }
 
 // ---- exclusive scan of uint32 (in place), three kernels
@@ -127,10 +130,12 @@ static hipError_t exclusive_scan(uint32_t* x, int64_t n, uint32_t* sums, hipStre
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift, int64_t tiles,
-    const uint32_t* __restrict__ start, K* __restrict__ kout, uint32_t* __restrict__ vout) {
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ tile_counts, K* __restrict__ kout,
+    uint32_t* __restrict__ vout) {
   __shared__ K skey[kTileKeys];
   __shared__ uint32_t sval[kTileKeys];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
+  __shared__ uint32_t gpos[256];       // global start of each digit's run of this tile
   __shared__ uint32_t run[256];        // next local slot of each digit
   __shared__ uint32_t wcnt[4][256];    // per-wave digit counts of the current chunk
   __shared__ uint32_t scan_tmp[256];
@@ -139,18 +144,12 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kTileKeys;
   const int tile_n = static_cast<int>(min<int64_t>(kTileKeys, n - base));
 
-  // tile histogram -> local digit starts
-  run[threadIdx.x] = 0;
-  __syncthreads();
-  for (int i = 0; i < kSortItems; ++i) {
-    const int jj = i * kSortThreads + threadIdx.x;
-    if (jj < tile_n) atomicAdd(&run[(kin[base + jj] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
+  // local digit starts from the tile's histogram (digit_hist), global starts from the scan
   uint32_t ex;
-  block_exscan256(scan_tmp, run[threadIdx.x], &ex);
+  block_exscan256(scan_tmp, tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x], &ex);
   loc[threadIdx.x] = ex;
   run[threadIdx.x] = ex;
+  gpos[threadIdx.x] = start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
   __syncthreads();
 
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -186,7 +185,7 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   for (int jj = threadIdx.x; jj < tile_n; jj += kSortThreads) {
     const K key = skey[jj];
     const uint32_t d = static_cast<uint32_t>((key >> shift) & 255u);
-    const uint32_t pos = start[static_cast<int64_t>(d) * tiles + blockIdx.x] + (jj - loc[d]);
+    const uint32_t pos = gpos[d] + (jj - loc[d]);
     kout[pos] = key;
     vout[pos] = sval[jj];
   }
@@ -198,7 +197,7 @@ size_t radix_sort_workspace(int64_t n, size_t key_bytes) {
   const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
   const int64_t m = tiles * 256;
   const int64_t chunks = (m + kScanChunk - 1) / kScanChunk;
-  return al(n * key_bytes) + al(n * 4) + al(m * 4) + al(chunks * 4);
+  return al(n * key_bytes) + al(n * 4) + 2 * al(m * 4) + al(chunks * 4);
 }
 
 template <typename K>
@@ -214,6 +213,8 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
   w += al(n * 4);
   uint32_t* counts = reinterpret_cast<uint32_t*>(w);
   w += al(m * 4);
+  uint32_t* tile_counts = reinterpret_cast<uint32_t*>(w);
+  w += al(m * 4);
   uint32_t* sums = reinterpret_cast<uint32_t*>(w);
   const int passes = (end_bit - begin_bit + 7) / 8;
   const K* ksrc = keys_in;
@@ -225,11 +226,11 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
     K* kdst = to_out ? keys_out : ktmp;
     uint32_t* vdst = to_out ? vals_out : vtmp;
     hipLaunchKernelGGL((digit_hist_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
-                       ksrc, n, shift, tiles, counts);
+                       ksrc, n, shift, tiles, counts, tile_counts);
     hipError_t e = exclusive_scan(counts, m, sums, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((digit_scatter_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
-                       ksrc, vsrc, n, shift, tiles, counts, kdst, vdst);
+                       ksrc, vsrc, n, shift, tiles, counts, tile_counts, kdst, vdst);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     ksrc = kdst;

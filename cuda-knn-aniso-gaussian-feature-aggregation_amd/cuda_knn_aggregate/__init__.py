@@ -212,6 +212,9 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         ctx.has_csr = csr is not None
         ctx.has_cache = cache is not None
         ctx.key = key
+        # outputs nothing depends on arrive as None (not zeros): the second order then
+        # skips their terms, and the Eikonal loss runs the dq-only kernels
+        ctx.set_materialize_grads(False)
         return dq, dp, df, dsigma, dR
 
     @staticmethod

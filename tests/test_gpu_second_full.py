@@ -101,3 +101,18 @@ def test_reference_second_order_switch(cuda):
     finally:
         ck.FULL_SECOND_ORDER = True
     assert plain.rel_err(full[2].cpu(), trunc[2].cpu()) > 1e-3    # the dp upstream matters for df
+
+
+def test_eikonal_step_runs_the_dq_only_second_order(cuda):
+    """The Eikonal loss depends on dq alone: the point outputs' upstream arrives as
+    None, so the second order runs the reference-order kernels, not the full ones."""
+    from cuda_knn_aggregate import _lib
+    b = workloads.clustered_batch(Q=2000, P=1500, K=8, F=8, seed=35).to(cuda)
+    params = [t.clone().requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    with _lib.trace() as tr:
+        f_out, *_ = knn_aggregate_aniso(*params, 8)
+        dq = torch.autograd.grad(f_out, params[0], b.v.expand_as(f_out), create_graph=True)[0]
+        L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+        torch.autograd.grad(L, params)
+    names = {n for n, _ in tr.records}
+    assert "point_grad2" in names and not any(n.endswith("_full") for n in names), names
