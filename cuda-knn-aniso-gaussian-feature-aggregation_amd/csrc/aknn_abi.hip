@@ -232,6 +232,7 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
 // Default search: the tiled search once clouds hold enough points for pruning to pay
 // (measured crossover ~8K points per cloud), the brute-force scan below that.
 int pick_search(const aknn_dims* d, const aknn_tune* t) {
+  if (t && t->index) return AKNN_SEARCH_TILES;  // a prebuilt point index is for the tiled search
   if (t && (t->search == AKNN_SEARCH_BRUTE || t->search == AKNN_SEARCH_TILES)) return t->search;
   const int64_t mean_p = d->B > 0 ? point_rows(d) / d->B : 0;
   return mean_p >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
@@ -259,6 +260,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.ws = ws;
     PL.ws_bytes = ws_bytes;
     PL.stats = tune ? static_cast<unsigned long long*>(tune->stats) : nullptr;
+    PL.index = tune ? tune->index : nullptr;
     if (tune && tune->point_split > 0) {
       int w = 1;
       while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;
@@ -474,6 +476,44 @@ int aknn_aggregate(const aknn_dims* d, const void* q, const void* p, const void*
     e = aknn::launch_aggregate(L, static_cast<hipStream_t>(stream));
   }
   if (e != hipSuccess) return hip_fail(e, "aggregate");
+  return AKNN_OK;
+}
+
+size_t aknn_index_size(const aknn_dims* d) {
+  if (check_dims(d, false) != AKNN_OK) return 0;
+  return aknn::pruned_index_bytes(pruned_launch(d));
+}
+
+size_t aknn_index_workspace_size(const aknn_dims* d) {
+  if (check_dims(d, false) != AKNN_OK) return 0;
+  return aknn::pruned_index_build_ws(pruned_launch(d));
+}
+
+int aknn_index_build(const aknn_dims* d, const void* p, const void* sigma, const void* R, void* index,
+                     size_t index_bytes, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_dims(d, false);
+  if (rc) return rc;
+  if (point_rows(d) == 0 || d->B == 0) return fail(AKNN_E_INVALID, "no points to index");
+  if (!p || !index || (d->key == AKNN_KEY_ANISO && (!sigma || !R)))
+    return fail(AKNN_E_INVALID, "null pointer");
+  aknn::PrunedLaunch PL = pruned_launch(d);
+  if (PL.ntiles >= (int64_t(1) << 30)) return fail(AKNN_E_UNSUPPORTED, "too many point tiles");
+  const size_t need = aknn::pruned_index_bytes(PL);
+  if (index_bytes < need) return fail(AKNN_E_WORKSPACE, "index buffer too small (%zu < %zu bytes)", index_bytes, need);
+  const size_t wneed = aknn::pruned_index_build_ws(PL);
+  if (!ws || ws_bytes < wneed) return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes, wneed);
+  PL.p = p;
+  PL.sigma = sigma;
+  PL.R = R;
+  PL.index = index;
+  PL.ws = ws;
+  PL.ws_bytes = ws_bytes;
+  hipError_t e;
+  {
+    Span sp("index_build", static_cast<hipStream_t>(stream));
+    e = aknn::launch_pruned_index_build(PL, static_cast<hipStream_t>(stream));
+  }
+  if (e != hipSuccess) return hip_fail(e, "index_build");
   return AKNN_OK;
 }
 

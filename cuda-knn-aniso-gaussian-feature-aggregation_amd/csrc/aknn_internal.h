@@ -39,6 +39,7 @@ struct PrunedLaunch {
   int64_t* idxs;
   void* ws;
   size_t ws_bytes;
+  const void* index;      // prebuilt point index (pruned_index_bytes), or null: built per call
 };
 
 // Gather-side (query rows) work: forward aggregation, first- and second-order dq.
@@ -98,6 +99,9 @@ hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
 hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st);
 hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st);
 hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes);
+size_t pruned_index_bytes(const PrunedLaunch& L);
+size_t pruned_index_build_ws(const PrunedLaunch& L);
+hipError_t launch_pruned_index_build(const PrunedLaunch& L, hipStream_t st);  // L.index = destination
 hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st);
 hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st);
 // spatial orders of the query and point rows (valid after the prep, inside the workspace)

@@ -783,26 +783,46 @@ struct Carver {
   }
 };
 
-static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
-  PrunedState S{};
-  Carver c{static_cast<char*>(L.ws)};
+// The point index: the cloud-only pieces (boxes, point order, tile records and
+// headers).  Carved from L.index when the caller keeps a prebuilt one (resident
+// clouds), else from the workspace and rebuilt by every call.
+static void carve_index(const PrunedLaunch& L, Carver& c, PrunedState& S) {
   const size_t el = L.dtype == AKNN_F64 ? 8 : 4;
   S.box = reinterpret_cast<float*>(c.take(L.cl.B * 8 * 4));
   S.obox = reinterpret_cast<unsigned*>(c.take(L.cl.B * 6 * 4));
-  S.pkeys = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
-  S.pkeys_s = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
   S.pperm = reinterpret_cast<uint32_t*>(c.take(L.prows * 4));
-  S.qkeys = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
-  S.qkeys_s = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
-  S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
   S.ptp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
-  S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.stp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.rec = c.take(L.ntiles * kTile * 16 * el);
   S.hdr = reinterpret_cast<float*>(c.take(L.ntiles * 8 * 4));
   S.shdr = reinterpret_cast<float*>(c.take(L.nsuper * 8 * 4));
   S.sub = reinterpret_cast<float*>(c.take(L.ntiles * 4 * 8 * 4));
   S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
+}
+
+size_t pruned_index_bytes(const PrunedLaunch& L) {
+  Carver c{nullptr};
+  PrunedState S{};
+  carve_index(L, c, S);
+  return c.used;
+}
+
+static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
+  PrunedState S{};
+  Carver c{static_cast<char*>(L.ws)};
+  const size_t el = L.dtype == AKNN_F64 ? 8 : 4;
+  if (L.index) {
+    Carver ci{static_cast<char*>(const_cast<void*>(L.index))};
+    carve_index(L, ci, S);
+  } else {
+    carve_index(L, c, S);
+  }
+  S.pkeys = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
+  S.pkeys_s = reinterpret_cast<uint64_t*>(c.take(L.prows * 8));
+  S.qkeys = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
+  S.qkeys_s = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
+  S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
+  S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
   if (L.ways > 1) {
@@ -861,11 +881,11 @@ __global__ void dense_prefix_kernel(int64_t per_cloud, int64_t* __restrict__ out
   }
 }
 
-// Boxes, Morton codes, both sorts, tile records/headers and super-tile headers.
+// The point index: boxes, point Morton codes and order, tile records/headers and
+// super-tile headers.
 template <typename T, int KEY>
-static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+static hipError_t prep_points(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
   const int bits = key_bits64(L.cl.B);
-  const T* q = static_cast<const T*>(L.q);
   const T* p = static_cast<const T*>(L.p);
   const int bs = 256;
   hipLaunchKernelGGL(box_init_kernel, dim3(static_cast<unsigned>((L.cl.B * 6 + bs - 1) / bs)), dim3(bs), 0, st,
@@ -877,17 +897,12 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
                      S.obox, L.cl.B, S.box);
   hipLaunchKernelGGL((codes_kernel<T, false>), dim3(static_cast<unsigned>((L.prows + bs - 1) / bs)), dim3(bs), 0,
                      st, p, L.cl, L.prows, S.box, S.pkeys);
-  hipLaunchKernelGGL((codes_kernel<T, true>), dim3(static_cast<unsigned>((L.qrows + bs - 1) / bs)), dim3(bs), 0,
-                     st, q, L.cl, L.qrows, S.box, S.qkeys);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = radix_sort_pairs<uint64_t>(S.pkeys, S.pkeys_s, nullptr, S.pperm, L.prows, 0, bits, S.tmp, st);
   if (e != hipSuccess) return e;
-  e = radix_sort_pairs<uint64_t>(S.qkeys, S.qkeys_s, nullptr, S.qperm, L.qrows, 0, bits, S.tmp, st);
-  if (e != hipSuccess) return e;
   if (L.cl.p_off) {
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.p_off, L.cl.B, int64_t(kTile), S.ptp);
-    hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.q_off, L.cl.B, int64_t(64), S.qtp);
     hipLaunchKernelGGL(super_prefix_kernel, dim3(1), dim3(64), 0, st, S.ptp, L.cl.B, S.stp);
   } else {
     const int64_t per = (L.cl.P + kTile - 1) / kTile;
@@ -900,6 +915,32 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
   hipLaunchKernelGGL(super_build_kernel, dim3(static_cast<unsigned>((L.nsuper + bs - 1) / bs)), dim3(bs), 0, st,
                      L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr);
   return hipGetLastError();
+}
+
+// Query side of every call: Morton codes in the cloud boxes, query order, tile prefix.
+template <typename T>
+static hipError_t prep_queries(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+  const int bits = key_bits64(L.cl.B);
+  const T* q = static_cast<const T*>(L.q);
+  const int bs = 256;
+  hipLaunchKernelGGL((codes_kernel<T, true>), dim3(static_cast<unsigned>((L.qrows + bs - 1) / bs)), dim3(bs), 0,
+                     st, q, L.cl, L.qrows, S.box, S.qkeys);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = radix_sort_pairs<uint64_t>(S.qkeys, S.qkeys_s, nullptr, S.qperm, L.qrows, 0, bits, S.tmp, st);
+  if (e != hipSuccess) return e;
+  if (L.cl.q_off)
+    hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.q_off, L.cl.B, int64_t(64), S.qtp);
+  return hipGetLastError();
+}
+
+template <typename T, int KEY>
+static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+  if (!L.index) {
+    const hipError_t e = prep_points<T, KEY>(L, S, st);
+    if (e != hipSuccess) return e;
+  }
+  return prep_queries<T>(L, S, st);
 }
 
 static PrunedState carve_run(const PrunedLaunch& L) {
@@ -925,6 +966,32 @@ hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st) {
                                        : launch_pruned_topk<float, AKNN_KEY_ANISO>(L, S, st);
   return L.key == AKNN_KEY_EUCLIDEAN ? launch_pruned_topk<double, AKNN_KEY_EUCLIDEAN>(L, S, st)
                                      : launch_pruned_topk<double, AKNN_KEY_ANISO>(L, S, st);
+}
+
+// Workspace of an index build: the point codes and the sort's temporaries.
+size_t pruned_index_build_ws(const PrunedLaunch& Lin) {
+  PrunedLaunch L = Lin;
+  L.qrows = 0;
+  L.ways = 1;
+  L.ws = nullptr;
+  L.index = reinterpret_cast<const void*>(uintptr_t(256));  // only sizes are taken
+  size_t tmp = 0;
+  (void)sort_temp(L, &tmp);
+  return carve(L, tmp).total;
+}
+
+// Builds the point index: L.index is the destination, L.ws the temporaries.
+hipError_t launch_pruned_index_build(const PrunedLaunch& Lin, hipStream_t st) {
+  PrunedLaunch L = Lin;
+  L.qrows = 0;
+  L.ways = 1;
+  const PrunedState S = carve_run(L);
+  if (S.total > L.ws_bytes) return hipErrorInvalidValue;
+  if (L.dtype == AKNN_F32)
+    return L.key == AKNN_KEY_EUCLIDEAN ? prep_points<float, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                       : prep_points<float, AKNN_KEY_ANISO>(L, S, st);
+  return L.key == AKNN_KEY_EUCLIDEAN ? prep_points<double, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                     : prep_points<double, AKNN_KEY_ANISO>(L, S, st);
 }
 
 const uint32_t* pruned_query_order(const PrunedLaunch& L) { return carve_run(L).qperm; }

@@ -27,6 +27,7 @@ __all__ = [
     "knn_aggregate_aniso_backward_2nd",
     "knn_aniso",
     "build_csr",
+    "build_index",
     "make_dims",
     "new_cache",
 ]
@@ -121,6 +122,8 @@ def _tune(tune) -> Optional[_lib.Tune]:
     t.search = _lib.SEARCH[tune.get("search", "auto")]
     st = tune.get("stats")  # optional int64 device tensor of 5 counters
     t.stats = st.data_ptr() if st is not None else None
+    ix = tune.get("index")  # optional point index from build_index (same points)
+    t.index = ix.data_ptr() if ix is not None else None
     return t
 
 
@@ -305,6 +308,36 @@ def knn_aggregate_forward(q, p, f, sigma, K, *, tune=None):
                                         scale.data_ptr(), f_out.data_ptr(), w_out.data_ptr(), dists.data_ptr(),
                                         idxs.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
     return [f_out, w_out, dists, idxs]
+
+
+def build_index(p, sigma, R, *, key: str = "aniso", offsets=None):
+    """Point index of a resident cloud batch (``p`` [B,P,3], ``sigma``, ``R``; or packed
+    ragged rows with ``offsets`` = (q_off, p_off, q_off_host, p_off_host)) as a uint8
+    device tensor: pass it as ``tune={"index": ...}`` to the forward / search of the same
+    points and the tiled search skips its per-call point preparation (same results)."""
+    for t, n in zip((p, sigma, R), ("p", "sigma", "R")):
+        _check_input(t, n)
+    dt = _check_float((p, sigma, R), ("p", "sigma", "R"))
+    if offsets is None:
+        if p.dim() != 3 or p.shape[2] != 3:
+            raise RuntimeError(f"p must have shape [B, P, 3] (got {tuple(p.shape)})")
+        B, P = p.shape[0], p.shape[1]
+        dims = make_dims(B, 1, P, 1, 1, dt, key)
+    else:
+        B = int(offsets[2].numel()) - 1
+        dims = make_dims(B, int(offsets[2][-1]), int(p.shape[0]), 1, 1, dt, key, offsets=offsets)
+    lib = _lib.load()
+    nbytes = lib.aknn_index_size(ctypes.byref(dims))
+    wbytes = lib.aknn_index_workspace_size(ctypes.byref(dims))
+    if nbytes == 0 or wbytes == 0:
+        raise RuntimeError(f"aknn_index_size failed: {lib.aknn_last_error().decode()}")
+    dev = p.device
+    index = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    ws = torch.empty(wbytes, dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.aknn_index_build(ctypes.byref(dims), p.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                                        index.data_ptr(), nbytes, ws.data_ptr(), wbytes, _stream(dev)))
+    return index
 
 
 def knn_aniso(q, p, sigma, R, K, *, key: str = "aniso", tune=None):

@@ -52,6 +52,9 @@ EXPORTS = (
     "aknn_backward",
     "aknn_backward2",
     "aknn_backward2_full",
+    "aknn_index_size",
+    "aknn_index_workspace_size",
+    "aknn_index_build",
     "aknn_lookup_workspace_size",
     "aknn_lookup_forward",
     "aknn_iso_workspace_size",
@@ -88,7 +91,8 @@ class Tune(ctypes.Structure):
     """``aknn_tune`` (include/aknn.h)."""
 
     _fields_ = [("queries_per_lane", ctypes.c_int32), ("point_split", ctypes.c_int32),
-                ("search", ctypes.c_int32), ("reserved", ctypes.c_int32), ("stats", ctypes.c_void_p)]
+                ("search", ctypes.c_int32), ("reserved", ctypes.c_int32), ("stats", ctypes.c_void_p),
+                ("index", ctypes.c_void_p)]
 
 
 SEARCH = {"auto": 0, "brute": 1, "tiles": 2}
@@ -119,6 +123,9 @@ _SIGS = {
                        + [_P] * 6 + [_P]),
     "aknn_backward2_full": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 14 + [_CACHE, _P, ctypes.c_size_t]
                             + [_P] * 6 + [_P]),
+    "aknn_index_size": (ctypes.c_size_t, [_DIMS]),
+    "aknn_index_workspace_size": (ctypes.c_size_t, [_DIMS]),
+    "aknn_index_build": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P]),
     "aknn_lookup_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_lookup_forward": (ctypes.c_int, [_DIMS] + [_P] * 6 + [ctypes.c_int64] + [_P] * 4
                             + [_P, ctypes.c_size_t, _P]),

@@ -72,6 +72,10 @@ class Server:
             self.clouds.append(tuple(t.to(self.device).contiguous()[None] for t in (p, f, s, R)))
         self.F = self.clouds[0][1].shape[-1]
         self.v = v.to(self.device, self.clouds[0][1].dtype)
+        # resident search indexes: the per-request search skips the point preparation
+        self.indexes = [_C.build_index(p, s, R, key=key) for p, _, s, R in self.clouds]
+        # the read-out row v for every query slot of the largest batch, built once
+        self.g_all = self.v.expand(int(max_batch_queries), self.F).contiguous()
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
         self.free_slots = list(range(streams))
         self.hbm_budget = int(hbm_budget)
@@ -116,10 +120,12 @@ class Server:
             r.started = t
         with torch.cuda.stream(st):
             qs = [r.q.to(self.device, non_blocking=True) for r in reqs]
-            q = torch.cat(qs, 0)[None].contiguous()
+            q = (torch.cat(qs, 0) if len(qs) > 1 else qs[0])[None].contiguous()
+            tune = dict(self.tune or {})
+            tune["index"] = self.indexes[reqs[0].cloud]
             f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, self.K, key=self.key,
-                                                                       tune=self.tune)
-            g = self.v.expand(1, q.shape[1], self.F).contiguous()
+                                                                       tune=tune)
+            g = self.g_all[: q.shape[1]][None]
             (dq,) = _C.knn_aggregate_aniso_backward(g, w_out, q, p, f, s, R, w_out, idxs, key=self.key,
                                                    needs=(True, False, False, False, False))[:1]
             f_host = torch.empty(f_out.shape[1:], dtype=f_out.dtype, pin_memory=True)

@@ -104,7 +104,23 @@ typedef struct aknn_tune {
   void* stats;              /* tiles: optional device uint64[5] the search adds its counts
                                to (super-tile tests, tile tests, tiles scanned, points
                                scanned, waves); NULL = off                               */
+  const void* index;        /* optional point index from aknn_index_build for these
+                               points (same dims except the queries): the tiled search
+                               then skips its per-call point preparation              */
 } aknn_tune;
+
+/* Point index (no counterpart in the reference; for callers that keep clouds resident
+ * and search them many times, like a server).  The tiled search's per-call point
+ * preparation -- cloud boxes, point Morton order, tile records and bounds -- built once
+ * into a caller-owned device buffer of aknn_index_size bytes (workspace for the build:
+ * aknn_index_workspace_size).  Pass it in aknn_tune.index to aknn_forward / aknn_knn
+ * with the same points, sigma, R, B, P, offsets, dtype and key mode; results are the
+ * same bits as without it.  Rebuild it when the points change. */
+AKNN_API size_t aknn_index_size(const aknn_dims* dims);
+AKNN_API size_t aknn_index_workspace_size(const aknn_dims* dims);
+AKNN_API int aknn_index_build(const aknn_dims* dims, const void* p, const void* sigma, const void* R,
+                              void* index, size_t index_bytes, void* workspace, size_t workspace_bytes,
+                              void* stream);
 
 /* Optional per-batch cache that aknn_forward fills and the backward passes read (no
  * counterpart in the reference; for callers that keep state between the passes, like

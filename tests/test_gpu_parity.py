@@ -358,3 +358,19 @@ def test_csr_transpose(cuda, P, Q, K):
     assert torch.equal(key, torch.sort(idxs.reshape(-1), stable=True).values)
     ref = torch.sort(idxs.reshape(-1), stable=True).indices
     assert torch.equal(ent, ref)
+
+
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_point_index_gives_the_same_results(cuda, key, dtype):
+    """A prebuilt point index (resident cloud) reused for several query sets: the
+    same bits as the per-call preparation, small (split walk) and large query sets."""
+    b = workloads.clustered_batch(Q=40000, P=20000, K=16, F=8, seed=41)
+    b = workloads.Batch(*(t.to(dtype) for t in (b.q, b.p, b.f, b.sigma, b.R)), b.K).to(cuda)
+    index = _C.build_index(b.p, b.sigma, b.R, key=key)
+    for sl in (slice(0, 300), slice(0, 40000), slice(1000, 9000)):
+        q = b.q[:, sl].contiguous()
+        ref = _C.knn_aggregate_aniso_forward(q, b.p, b.f, b.sigma, b.R, 16, key=key, tune={"search": "tiles"})
+        got = _C.knn_aggregate_aniso_forward(q, b.p, b.f, b.sigma, b.R, 16, key=key, tune={"index": index})
+        for x, y in zip(got, ref):
+            assert torch.equal(x, y)

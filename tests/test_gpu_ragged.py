@@ -146,3 +146,16 @@ def test_graph_capture_replays_match_eager(cuda, ragged):
             assert torch.equal(x, y), rep
         junk = torch.randn(1 << 20, device=cuda) * 2.0  # eager work between replays
         junk.sum().item()
+
+
+def test_ragged_point_index(cuda):
+    """A point index built over the packed clouds reproduces the per-call search."""
+    r = small_ragged(6)
+    layout = RaggedLayout(r.q_off, r.p_off, cuda)
+    d = r.to(cuda)
+    offs = (layout.q_off, layout.p_off, layout.q_off_host, layout.p_off_host)
+    index = _C.build_index(d.p, d.sigma, d.R, offsets=offs)
+    ref = rg.forward(layout, d.q, d.p, d.f, d.sigma, d.R, r.K, tune={"search": "tiles"})
+    got = rg.forward(layout, d.q, d.p, d.f, d.sigma, d.R, r.K, tune={"index": index})
+    for x, y in zip(got, ref):
+        assert torch.equal(x, y)
