@@ -251,7 +251,7 @@ def serving_bench(args):
     server.warm_up()                            # code objects, device and pinned-host allocator caches
     server.replay(requests(100, 500.0, 1))
     rows = []
-    for rate in (250.0, 500.0, 1000.0, 2000.0, 4000.0, 8000.0):
+    for rate in (250.0, 500.0, 1000.0, 2000.0, 3000.0, 4000.0, 6000.0, 8000.0):
         n = int(min(2000, max(200, rate * 2.0)))
         reqs = requests(n, rate, int(rate))
         b0 = server.batches

@@ -233,9 +233,28 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
 // (measured crossover ~8K points per cloud), the brute-force scan below that.
 int pick_search(const aknn_dims* d, const aknn_tune* t) {
   if (t && t->index) return AKNN_SEARCH_TILES;  // a prebuilt point index is for the tiled search
+  if (t && t->search == AKNN_SEARCH_WAVE) return AKNN_SEARCH_TILES;  // same index, per-query walk
   if (t && (t->search == AKNN_SEARCH_BRUTE || t->search == AKNN_SEARCH_TILES)) return t->search;
   const int64_t mean_p = d->B > 0 ? point_rows(d) / d->B : 0;
   return mean_p >= 8192 ? AKNN_SEARCH_TILES : AKNN_SEARCH_BRUTE;
+}
+
+// Within the tiled search: the per-query walk when the queries are sparse in their
+// clouds -- at most one per point of the clouds that have queries -- where the 64
+// queries of a query tile share few point tiles.  Measured on 64K-point clustered
+// clouds: 5x faster at 256 queries, 1.3x at 64K, 2x slower at 256K.
+
+bool pick_wave(const aknn_dims* d, const aknn_tune* t) {
+  if (t && t->search == AKNN_SEARCH_WAVE) return true;
+  if (t && t->search == AKNN_SEARCH_TILES) return false;
+  int64_t pts = 0;
+  if (ragged(d)) {
+    for (int64_t b = 0; b < d->B; ++b)
+      if (d->q_off_host[b + 1] > d->q_off_host[b]) pts += d->p_off_host[b + 1] - d->p_off_host[b];
+  } else if (d->Q > 0) {
+    pts = d->B * d->P;
+  }
+  return query_rows(d) <= pts;
 }
 
 // Neighbour search.  On return *qorder / *porder point at spatial orders of the query
@@ -261,6 +280,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.ws_bytes = ws_bytes;
     PL.stats = tune ? static_cast<unsigned long long*>(tune->stats) : nullptr;
     PL.index = tune ? tune->index : nullptr;
+    PL.wave = pick_wave(d, tune) ? 1 : 0;
     if (tune && tune->point_split > 0) {
       int w = 1;
       while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;
@@ -273,7 +293,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     }
     if (e != hipSuccess) return hip_fail(e, "knn_tiles_prep");
     {
-      Span sp("knn_tiles", st);
+      Span sp(PL.wave ? "knn_wave" : "knn_tiles", st);
       e = aknn::launch_knn_pruned_search(PL, st);
     }
     if (e != hipSuccess) return hip_fail(e, "knn_tiles");

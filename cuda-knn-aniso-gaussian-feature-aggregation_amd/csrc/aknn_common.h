@@ -57,6 +57,32 @@ struct Clouds {
 // rows) remap the hardware block id so each XCD works through one contiguous range of
 // logical blocks.  A bijection on [0, n) for any n; results never depend on it.
 constexpr int kXcds = 8;
+
+// Point index of an unfilled top-K slot inside the search kernels.
+constexpr int kNoPoint = 0x7fffffff;
+
+// A query with fewer than K keys that compare (NaN query coordinates, or a cloud with
+// fewer than K points whose keys are not NaN) leaves the tail of its list unfilled.
+// Those slots take the lowest point indices not already in the list, with a NaN key:
+// every index stays inside the cloud, and a NaN query gets 0..K-1 with NaN keys, which
+// is what the reference's scan keeps when no key compares (csrc/mink.cuh:60-81 fills
+// its K slots with the first points scanned and never replaces them).  Called, rarely,
+// after the row's slots have been stored by the same thread.
+template <typename T>
+__device__ __noinline__ void fill_unfilled_row(T* __restrict__ d, int64_t* __restrict__ ix, int K) {
+  int64_t c = 0;
+  for (int s = 0; s < K; ++s) {
+    if (ix[s] != kNoPoint) continue;
+    for (;;) {
+      bool used = false;
+      for (int t = 0; t < K; ++t) used |= ix[t] == c;
+      if (!used) break;
+      ++c;
+    }
+    ix[s] = c++;
+    d[s] = static_cast<T>(__builtin_nan(""));
+  }
+}
 __device__ __forceinline__ int64_t xcd_block(int64_t hw, int64_t n) {
   const int64_t q = n / kXcds, r = n % kXcds;
   const int64_t x = hw % kXcds, i = hw / kXcds;

@@ -40,6 +40,7 @@ struct PrunedLaunch {
   void* ws;
   size_t ws_bytes;
   const void* index;      // prebuilt point index (pruned_index_bytes), or null: built per call
+  int32_t wave;           // per-query walk (one wave per query) instead of query tiles
 };
 
 // Gather-side (query rows) work: forward aggregation, first- and second-order dq.

@@ -25,7 +25,7 @@
 
 namespace aknn {
 
-constexpr int kSentinel = 0x7fffffff;
+constexpr int kSentinel = kNoPoint;
 
 template <typename T>
 __device__ __forceinline__ T pos_inf();
@@ -184,13 +184,16 @@ __global__ __launch_bounds__(64 * SPLIT) void knn_topk_kernel(
   for (int i = 0; i < QPT; ++i) {
     const int64_t row = q0 + lane + 64 * i;
     if (row < qend) {
+      int last = 0;
 #pragma unroll
       for (int s = 0; s < KMAX; ++s) {
         if (s < K) {
           dists[row * K + s] = key[i][s];
           idxs[row * K + s] = static_cast<int64_t>(id[i][s]);
+          last = id[i][s];
         }
       }
+      if (last == kNoPoint) fill_unfilled_row<T>(dists + row * K, idxs + row * K, K);
     }
   }
 }

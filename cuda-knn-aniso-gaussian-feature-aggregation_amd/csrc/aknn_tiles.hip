@@ -30,7 +30,7 @@ namespace aknn {
 
 constexpr int kTile = 64;
 constexpr int kSuper = 16;  // tiles per super-tile
-constexpr int kTileSentinel = 0x7fffffff;
+constexpr int kTileSentinel = kNoPoint;
 
 // Workspace pieces of the tiled search (see carve()).
 struct PrunedState {
@@ -588,13 +588,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
 
   if (valid) {
     if (ways == 1) {
+      int last = 0;
 #pragma unroll
       for (int s = 0; s < KMAX; ++s) {
         if (s < K) {
           dists[row * K + s] = key[s];
           idxs[row * K + s] = static_cast<int64_t>(id[s]);
+          last = id[s];
         }
       }
+      if (last == kNoPoint) fill_unfilled_row<T>(dists + row * K, idxs + row * K, K);
     } else {
       T* lk = lkey + (pos * ways + way) * K;
       int32_t* li = lid + (pos * ways + way) * K;
@@ -664,10 +667,62 @@ __global__ void super_build_kernel(Clouds cl, const int64_t* __restrict__ ptiles
   o[7] = 0.f;
 }
 
-template <typename U>
-__global__ void bound_fill_kernel(U* __restrict__ bnd, int64_t n, U v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) bnd[i] = v;
+// Split walk: the shared bound of each sorted query position starts at the K-th
+// smallest key over the 64 points of the tile its Morton code falls in (+inf if that
+// tile holds fewer than K finite keys).  Those are real points of the query's cloud, so
+// the value bounds the final K-th key from above and every tile the walk skips against
+// it holds no point of the result; the walk itself inserts nothing from here (no
+// duplicates).  Queries far from the walk's seed tile then prune from the first step
+// instead of scanning with an infinite bound until the walk reaches them.
+template <typename T, int KMAX, int KEY>
+__global__ __launch_bounds__(256) void seed_bound_kernel(
+    const T* __restrict__ q, Clouds cl, const uint32_t* __restrict__ qperm,
+    const uint64_t* __restrict__ qkeys, const int64_t* __restrict__ ptiles, const T* __restrict__ rec,
+    const float* __restrict__ hdr, int K, int64_t qrows, typename OrdKey<T>::U* __restrict__ bnd) {
+  using Ord = OrdKey<T>;
+  const int64_t pos = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pos >= qrows) return;
+  const int64_t b = cl.cloud_of_query(pos);
+  int64_t tb, te;
+  if (cl.p_off) {
+    tb = ptiles[b];
+    te = ptiles[b + 1];
+  } else {
+    tb = b * ptiles[1];
+    te = tb + ptiles[1];
+  }
+  T bound = __builtin_inf();
+  if (te > tb) {
+    const uint32_t code = static_cast<uint32_t>(qkeys[pos] & 0xffffffffu);
+    int64_t lo = tb, hi = te - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (__float_as_uint(hdr[mid * 8 + 7]) <= code) lo = mid; else hi = mid - 1;
+    }
+    const int64_t row = qperm[pos];
+    const T qx = q[row * 3 + 0], qy = q[row * 3 + 1], qz = q[row * 3 + 2];
+    T key[KMAX];
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) key[s] = __builtin_inf();
+    const T* __restrict__ base = rec + lo * (kTile * 16);
+#pragma unroll 4
+    for (int j = 0; j < kTile; ++j) {
+      const T c = rec_key<T, KEY>(qx, qy, qz, load_rec16(base + j * 16));
+      if (c < key[KMAX - 1]) {  // sorted insert of the value (NaN never enters)
+        T carry = c;
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+          const T lo_v = fmin(key[s], carry);
+          carry = fmax(key[s], carry);
+          key[s] = lo_v;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s)
+      if (s == K - 1) bound = key[s];
+  }
+  bnd[pos] = Ord::enc(bound);
 }
 
 // Split walk: merge the `ways` per-way top-K lists of each sorted query position.
@@ -718,25 +773,213 @@ __global__ __launch_bounds__(64) void pruned_merge_kernel(const T* __restrict__ 
   }
   if (ok && w == 0) {
     const int64_t row = qperm[pos];
+    int last = 0;
 #pragma unroll
     for (int s = 0; s < KMAX; ++s) {
       if (s < K) {
         dists[row * K + s] = key[s];
         idxs[row * K + s] = static_cast<int64_t>(id[s]);
+        last = id[s];
+      }
+    }
+    if (last == kNoPoint) fill_unfilled_row<T>(dists + row * K, idxs + row * K, K);
+  }
+}
+
+// ------------------------------------------------------------------ per-query walk
+// Sparse query sets (few queries per cloud point): the 64 queries of a Morton tile lie
+// far apart, so the tile walk scans the union of their neighbourhoods.  Here one wave
+// serves one query: it evaluates a whole point tile per step (lane j: record j), keeps
+// the query's top-K as a sorted (key, index) list in lanes 0..K-1, and visits
+// super-tiles and tiles best first (smallest certified lower bound first; "always
+// scan" boxes before all), each at most once, skipping those whose bound exceeds the
+// list's last key.  The result is the same exact top-K, in the same order.
+
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long x = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+  }
+}
+
+// Lowest lane among `sel` holding the smallest `v` (NaN ordered first); some lane must
+// be selected.
+template <typename T>
+__device__ __forceinline__ int wave_argmin(T v, bool sel) {
+  const T x = sel ? (v != v ? -__builtin_inf() : v) : __builtin_inf();
+  T m = x;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  return __builtin_ctzll(__ballot(sel && x == m));
+}
+
+// Certified lower bound of a box's keys for the query (negative infinity: "always
+// scan" box, whose mu is negative).
+template <typename T>
+__device__ __forceinline__ T box_bound(const float* __restrict__ h, T qx, T qy, T qz) {
+  const float4 a = *reinterpret_cast<const float4*>(h);
+  const float4 c = *reinterpret_cast<const float4*>(h + 4);
+  const T mu = static_cast<T>(c.z);
+  const T dx = fmax(fmax(static_cast<T>(a.x) - qx, qx - static_cast<T>(a.w)), T(0));
+  const T dy = fmax(fmax(static_cast<T>(a.y) - qy, qy - static_cast<T>(c.x)), T(0));
+  const T dz = fmax(fmax(static_cast<T>(a.z) - qz, qz - static_cast<T>(c.y)), T(0));
+  return mu < T(0) ? -__builtin_inf() : mu * (dx * dx + dy * dy + dz * dz);
+}
+
+template <typename T, int KEY>
+__global__ __launch_bounds__(256) void knn_wave_kernel(
+    const T* __restrict__ q, Clouds cl, int64_t qrows, const int64_t* __restrict__ ptiles,
+    const int64_t* __restrict__ stiles, const T* __restrict__ rec, const float* __restrict__ hdr,
+    const float* __restrict__ shdr, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
+    unsigned long long* __restrict__ stats) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= qrows) return;  // whole waves
+  const int64_t b = cl.cloud_of_query(row);
+  int64_t tb, te, sb, se;
+  if (cl.p_off) {
+    tb = ptiles[b];
+    te = ptiles[b + 1];
+    sb = stiles[b];
+    se = stiles[b + 1];
+  } else {
+    tb = b * ptiles[1];
+    te = tb + ptiles[1];
+    sb = b * stiles[1];
+    se = sb + stiles[1];
+  }
+  const T qx = q[row * 3 + 0], qy = q[row * 3 + 1], qz = q[row * 3 + 2];
+  // +inf reaches the wave-uniform values through a vector register: for a double kept
+  // in scalar registers hipcc (gfx950) emits s_mov_b64 with a 64-bit literal of which
+  // the encoding keeps the low half (+inf became 0.0)
+  T inf_v = __builtin_inf();
+  asm volatile("" : "+v"(inf_v));
+  T lk = inf_v;  // lanes 0..K-1: the sorted list
+  int li = kTileSentinel;
+  T last_k = lane_value<T>(inf_v, 0);
+  int last_i = kTileSentinel;
+  unsigned n_super = 0, n_tile = 0, n_scan = 0;
+
+  const int64_t nsup = se - sb;
+  const int64_t nchunk = (nsup + 63) / 64;
+  // chunks of 64 super-tiles: the one holding the closest super-tile first, then
+  // outward (a cloud of up to 64K points is one chunk)
+  int64_t first = 0;
+  if (nchunk > 1) {
+    T best = lane_value<T>(inf_v, 0);
+    int64_t bc = 0;
+    for (int64_t c = 0; c < nchunk; ++c) {
+      const int64_t s = sb + c * 64 + lane;
+      const T v = s < se ? box_bound<T>(shdr + s * 8, qx, qy, qz) : __builtin_inf();
+      const int l = wave_argmin<T>(v, s < se);
+      const T m = lane_value<T>(v, l);
+      if (m < best) {
+        best = m;
+        bc = c;
+      }
+    }
+    first = bc;
+  }
+  int64_t up = first, down = first - 1;
+  for (int64_t step = 0; step < nchunk; ++step) {
+    const bool take_up = (up < nchunk) && ((step & 1) == 0 || down < 0);
+    const int64_t c = take_up ? up++ : down--;
+    const int64_t s = sb + c * 64 + lane;
+    const bool sin = s < se;
+    const T slb = sin ? box_bound<T>(shdr + s * 8, qx, qy, qz) : __builtin_inf();
+    bool sdone = !sin;
+    while (true) {
+      const bool ssel = !sdone && !(slb > last_k);
+      if (!__any(ssel)) break;
+      const int sl = wave_argmin<T>(slb, ssel);
+      if (lane == sl) sdone = true;
+      ++n_super;
+      const int64_t t0 = tb + (c * 64 + sl) * kSuper;
+      const int64_t nt = te - t0 < kSuper ? te - t0 : kSuper;
+      const bool tin = lane < nt;
+      const T tlb = tin ? box_bound<T>(hdr + (t0 + lane) * 8, qx, qy, qz) : __builtin_inf();
+      bool tdone = !tin;
+      while (true) {
+        const bool tsel = !tdone && !(tlb > last_k);
+        if (!__any(tsel)) break;
+        const int tl = wave_argmin<T>(tlb, tsel);
+        if (lane == tl) tdone = true;
+        ++n_tile;
+        const Rec16<T> r = load_rec16(rec + (t0 + tl) * (kTile * 16) + lane * 16);
+        const T ck = rec_key<T, KEY>(qx, qy, qz, r);
+        const int ci = rec_index(r.v[15]);
+        unsigned long long m = __ballot((ck < last_k) | ((ck == last_k) & (ci < last_i)));
+        if (m) ++n_scan;
+        while (m) {
+          const int l = __builtin_ctzll(m);
+          m &= m - 1;
+          const T k = lane_value<T>(ck, l);
+          const int i = __builtin_amdgcn_readlane(ci, l);
+          if (!((k < last_k) | ((k == last_k) & (i < last_i)))) continue;
+          // insert into the sorted list: lanes at or after the insertion point shift up
+          const bool lt = (lane < K) & ((k < lk) | ((k == lk) & (i < li)));
+          const T pk = __shfl_up(lk, 1, 64);
+          const int pi = __shfl_up(li, 1, 64);
+          const bool plt = (lane > 0) & (__shfl_up(static_cast<int>(lt), 1, 64) != 0);
+          if (lt) {
+            lk = plt ? pk : k;
+            li = plt ? pi : i;
+          }
+          last_k = lane_value<T>(lk, K - 1);
+          last_i = __builtin_amdgcn_readlane(li, K - 1);
+        }
       }
     }
   }
+  if (last_i == kNoPoint) {
+    // unfilled tail (fewer than K keys compare): the lowest point indices not in the
+    // list, NaN keys, as fill_unfilled_row does for the other searches
+    bool used = false;
+    for (int j = 0; j < K; ++j) used |= __builtin_amdgcn_readlane(li, j) == lane;
+    unsigned long long free_idx = __ballot(!used && lane < K);
+    const int filled = __builtin_popcountll(__ballot(lane < K && li != kNoPoint));
+    if (lane < K && li == kNoPoint) {
+      for (int t = filled; t < lane; ++t) free_idx &= free_idx - 1;
+      li = __builtin_ctzll(free_idx);
+      lk = static_cast<T>(__builtin_nan(""));
+    }
+  }
+  if (lane < K) {
+    dists[row * K + lane] = lk;
+    idxs[row * K + lane] = static_cast<int64_t>(li);
+  }
+  if (stats && lane == 0) {
+    atomicAdd(stats + 0, static_cast<unsigned long long>(n_super));
+    atomicAdd(stats + 1, static_cast<unsigned long long>(n_tile));
+    atomicAdd(stats + 2, static_cast<unsigned long long>(n_scan));
+    atomicAdd(stats + 3, static_cast<unsigned long long>(n_tile) * kTile);
+    atomicAdd(stats + 4, 1ull);
+  }
+}
+
+template <typename T, int KEY>
+static hipError_t launch_wave(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
+  if (L.qrows == 0) return hipSuccess;
+  hipLaunchKernelGGL((knn_wave_kernel<T, KEY>), dim3(static_cast<unsigned>((L.qrows + 3) / 4)), dim3(256), 0,
+                     st, static_cast<const T*>(L.q), L.cl, L.qrows, S.ptp, S.stp, static_cast<const T*>(S.rec),
+                     S.hdr, S.shdr, static_cast<int>(L.K), static_cast<T*>(L.dists), L.idxs, L.stats);
+  return hipGetLastError();
 }
 
 template <typename T, int KMAX, int KEY>
 static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
   using U = typename OrdKey<T>::U;
   const int ways = L.ways > 1 ? L.ways : 1;
-  if (ways > 1) {
-    const U inf_enc = sizeof(T) == 4 ? U(0xff800000u) : U(0xfff0000000000000ull);  // enc(+inf)
-    hipLaunchKernelGGL((bound_fill_kernel<U>), dim3(static_cast<unsigned>((L.qrows + 255) / 256)), dim3(256),
-                       0, st, static_cast<U*>(S.bnd), L.qrows, inf_enc);
-  }
+  if (ways > 1)
+    hipLaunchKernelGGL((seed_bound_kernel<T, KMAX, KEY>), dim3(static_cast<unsigned>((L.qrows + 255) / 256)),
+                       dim3(256), 0, st, static_cast<const T*>(L.q), L.cl, S.qperm, S.qkeys_s, S.ptp,
+                       static_cast<const T*>(S.rec), S.hdr, static_cast<int>(L.K), L.qrows,
+                       static_cast<U*>(S.bnd));
   dim3 grid;
   if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles * ways), 1);
   else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64 * ways), static_cast<unsigned>(L.cl.B));
@@ -940,6 +1183,7 @@ static hipError_t prep_pruned(const PrunedLaunch& L, const PrunedState& S, hipSt
     const hipError_t e = prep_points<T, KEY>(L, S, st);
     if (e != hipSuccess) return e;
   }
+  if (L.wave) return hipSuccess;  // the per-query walk reads the queries as they are
   return prep_queries<T>(L, S, st);
 }
 
@@ -961,6 +1205,13 @@ hipError_t launch_knn_pruned_prep(const PrunedLaunch& L, hipStream_t st) {
 
 hipError_t launch_knn_pruned_search(const PrunedLaunch& L, hipStream_t st) {
   const PrunedState S = carve_run(L);
+  if (L.wave) {
+    if (L.dtype == AKNN_F32)
+      return L.key == AKNN_KEY_EUCLIDEAN ? launch_wave<float, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                         : launch_wave<float, AKNN_KEY_ANISO>(L, S, st);
+    return L.key == AKNN_KEY_EUCLIDEAN ? launch_wave<double, AKNN_KEY_EUCLIDEAN>(L, S, st)
+                                       : launch_wave<double, AKNN_KEY_ANISO>(L, S, st);
+  }
   if (L.dtype == AKNN_F32)
     return L.key == AKNN_KEY_EUCLIDEAN ? launch_pruned_topk<float, AKNN_KEY_EUCLIDEAN>(L, S, st)
                                        : launch_pruned_topk<float, AKNN_KEY_ANISO>(L, S, st);
@@ -994,7 +1245,7 @@ hipError_t launch_pruned_index_build(const PrunedLaunch& Lin, hipStream_t st) {
                                      : prep_points<double, AKNN_KEY_ANISO>(L, S, st);
 }
 
-const uint32_t* pruned_query_order(const PrunedLaunch& L) { return carve_run(L).qperm; }
+const uint32_t* pruned_query_order(const PrunedLaunch& L) { return L.wave ? nullptr : carve_run(L).qperm; }
 const uint32_t* pruned_point_order(const PrunedLaunch& L) { return carve_run(L).pperm; }
 
 __global__ void identity_kernel(int32_t* __restrict__ out, int64_t n) {

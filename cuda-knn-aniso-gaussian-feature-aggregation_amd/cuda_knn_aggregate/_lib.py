@@ -95,7 +95,7 @@ class Tune(ctypes.Structure):
                 ("index", ctypes.c_void_p)]
 
 
-SEARCH = {"auto": 0, "brute": 1, "tiles": 2}
+SEARCH = {"auto": 0, "brute": 1, "tiles": 2, "wave": 3}
 
 
 _lock = threading.Lock()

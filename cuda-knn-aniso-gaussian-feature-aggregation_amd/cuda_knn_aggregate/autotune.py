@@ -21,7 +21,7 @@ import torch
 from . import _C
 
 CANDIDATES = (
-    [{"search": "tiles"}]
+    [{"search": "tiles"}, {"search": "wave"}]
     + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]
 )
 

@@ -1,36 +1,40 @@
 """Continuous-batching request loop over resident point clouds.
 
-Clouds (points, features, Gaussians) stay resident in HBM.  Each request asks for the
-aggregated features ``f_out`` of its queries against one cloud and the gradient
-``n = d(f_out . v)/dq`` (the SDF-normal read-out, ``v`` fixed per server), i.e. the
-forward plus the query-side first-order backward.
+Clouds (points, features, Gaussians) stay resident in HBM, packed into one ragged point
+set with one search index.  Each request asks for the aggregated features ``f_out`` of
+its queries against one cloud and the gradient ``n = d(f_out . v)/dq`` (the SDF-normal
+read-out, ``v`` fixed per server), i.e. the forward plus the query-side first-order
+backward.
 
 Scheduling (one host thread, non-blocking):
 
 * arrived requests wait in FIFO order;
-* a batch is formed whenever a stream slot is free: the oldest waiting request picks
-  the cloud, and later waiting requests for the same cloud join while the batch stays
-  under ``max_batch_queries`` and the in-flight HBM estimate stays under
-  ``hbm_budget`` (admission control);
-* a batch is one dense launch sequence (search + aggregation + dq) on its own HIP
-  stream, so batches for different clouds overlap on the device; results come back
-  with an asynchronous device-to-host copy and a completion event;
+* a batch is formed whenever a stream slot is free: waiting requests join in arrival
+  order, whatever their clouds, while the batch stays under ``max_batch_queries`` and
+  the in-flight HBM estimate stays under ``hbm_budget`` (admission control);
+* a batch is one ragged launch sequence (search + aggregation + dq, every cloud of the
+  batch in the same launches, its queries grouped by cloud) on its own HIP stream, in
+  buffers preallocated for that stream slot, so batches overlap on the device; results
+  come back with an asynchronous device-to-host copy into pinned staging and a
+  completion event;
 * completed batches release their slot and budget at once, so new arrivals are admitted
   while other work is still on the GPU.
 
 Each row's result does not depend on the batch it is served in (exact search, per-row
-aggregation and gradient), so a request gets the same bits alone and in any mix.
+aggregation and gradient; a ragged launch gives each cloud's rows the bits of a dense
+launch on that cloud alone), so a request gets the same bits alone and in any mix.
 """
 from __future__ import annotations
 
 import collections
+import ctypes
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional
 
 import torch
 
-from . import _C
+from . import _C, _lib
 
 
 @dataclass
@@ -43,6 +47,7 @@ class Request:
     f_out: Optional[torch.Tensor] = None
     normal: Optional[torch.Tensor] = None
     rid: int = 0
+    row: int = 0                     # first row of its queries in the batch
 
     @property
     def latency(self) -> float:
@@ -55,31 +60,76 @@ class _Batch:
     event: torch.cuda.Event
     slot: int
     bytes: int
-    outs: list = field(default_factory=list)
+    nq: int = 0
+
+
+class _Slot:
+    """One stream slot: its HIP stream and the buffers of a batch of up to ``max_q``
+    queries (device outputs and workspace, query offsets, pinned host staging)."""
+
+    def __init__(self, device, dtype, F, K, max_q, n_clouds, ws_bytes):
+        self.stream = torch.cuda.Stream(device=device)
+        self.q = torch.empty(max_q, 3, dtype=dtype, device=device)
+        self.f_out = torch.empty(max_q, F, dtype=dtype, device=device)
+        self.w_out = torch.empty(max_q, K, dtype=dtype, device=device)
+        self.dists = torch.empty(max_q, K, dtype=dtype, device=device)
+        self.idxs = torch.empty(max_q, K, dtype=torch.int64, device=device)
+        self.dq = torch.empty(max_q, 3, dtype=dtype, device=device)
+        self.ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=device)
+        self.q_off_host = torch.zeros(n_clouds + 1, dtype=torch.int64, pin_memory=True)
+        self.q_off = torch.zeros(n_clouds + 1, dtype=torch.int64, device=device)
+        self.f_host = torch.empty(max_q, F, dtype=dtype, pin_memory=True)
+        self.n_host = torch.empty(max_q, 3, dtype=dtype, pin_memory=True)
+        self.event = torch.cuda.Event()
 
 
 class Server:
     def __init__(self, clouds, K, v, *, device="cuda:0", streams=4, hbm_budget=2 << 30,
                  max_batch_queries=131072, tune=None, key="aniso", on_done=None):
         """clouds: list of (p [P,3], f [P,F], sigma [P,3], R [P,3,3]) tensors (moved to
-        ``device`` and kept resident); v: [F] read-out for the normal.  ``on_done(req)``
-        is called as each request completes (a consumer that takes the results and
-        drops them lets their pinned host buffers return to the allocator's cache)."""
+        ``device``, packed and kept resident); v: [F] read-out for the normal.
+
+        ``on_done(req)`` is called as each request completes, with ``req.f_out`` and
+        ``req.normal`` as views into the slot's pinned staging buffers, valid until the
+        callback returns (copy what must outlive it).  Without a callback the results
+        are copied into each request.  Requests should hold their queries in pinned host
+        memory (``tensor.pin_memory()``) so the uploads run asynchronously."""
         self.device = torch.device(device)
-        self.K, self.key, self.tune = int(K), key, tune
-        self.clouds = []
-        for p, f, s, R in clouds:
-            self.clouds.append(tuple(t.to(self.device).contiguous()[None] for t in (p, f, s, R)))
-        self.F = self.clouds[0][1].shape[-1]
-        self.v = v.to(self.device, self.clouds[0][1].dtype)
-        # resident search indexes: the per-request search skips the point preparation
-        self.indexes = [_C.build_index(p, s, R, key=key) for p, _, s, R in self.clouds]
-        # the read-out row v for every query slot of the largest batch, built once
-        self.g_all = self.v.expand(int(max_batch_queries), self.F).contiguous()
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
-        self.free_slots = list(range(streams))
+        self.K, self.key = int(K), key
+        parts = [tuple(t.to(self.device).contiguous() for t in c) for c in clouds]
+        if not parts:
+            raise ValueError("no clouds")
+        self.F = parts[0][1].shape[-1]
+        self.dtype = parts[0][1].dtype
+        for p, f, s, R in parts:
+            if f.shape[-1] != self.F or f.dtype != self.dtype:
+                raise ValueError("all clouds must share the feature width and dtype")
+            if p.shape[0] < self.K:
+                raise ValueError("every cloud needs at least K points")
+        self.n_clouds = len(parts)
+        self.p, self.f, self.sigma, self.R = (torch.cat([c[i] for c in parts]).contiguous() for i in range(4))
+        sizes = torch.tensor([0] + [c[0].shape[0] for c in parts], dtype=torch.int64)
+        self.p_off_host = torch.cumsum(sizes, 0).contiguous()
+        self.p_off = self.p_off_host.to(self.device)
+        self.P = int(self.p_off_host[-1])
+        self.v = v.to(self.device, self.dtype)
         self.hbm_budget = int(hbm_budget)
         self.max_batch_queries = int(max_batch_queries)
+        self._lib = _lib.load()
+        # one search index for the packed clouds: the per-batch search skips the point
+        # preparation (built with an empty query side; the index holds points only)
+        q0 = torch.zeros(self.n_clouds + 1, dtype=torch.int64)
+        self.index = _C.build_index(self.p, self.sigma, self.R, key=key,
+                                    offsets=(q0.to(self.device), self.p_off, q0, self.p_off_host))
+        self.tune = _C._tune(tune) or _lib.Tune()
+        self.tune.index = self.index.data_ptr()
+        # the read-out row v for every query slot of the largest batch, built once
+        self.g_all = self.v.expand(self.max_batch_queries, self.F).contiguous()
+        ws = self._ws_bytes(self.max_batch_queries)
+        self.slots = [_Slot(self.device, self.dtype, self.F, self.K, self.max_batch_queries, self.n_clouds, ws)
+                      for _ in range(streams)]
+        self.streams = [sl.stream for sl in self.slots]
+        self.free_slots = list(range(streams))
         self.inflight: List[_Batch] = []
         self.in_use = 0
         self.waiting: collections.deque = collections.deque()
@@ -87,77 +137,114 @@ class Server:
         self.batches = 0
         self.on_done = on_done
 
+    def _dims(self, nq, q_off, q_off_host):
+        return _C.make_dims(self.n_clouds, nq, self.P, self.F, self.K, self.dtype, self.key,
+                            offsets=(q_off, self.p_off, q_off_host, self.p_off_host))
+
+    def _ws_bytes(self, nq) -> int:
+        """Forward workspace for nq queries, the largest over how they spread over the
+        clouds (all in one cloud, or evenly)."""
+        worst = 0
+        for spread in ("one", "even"):
+            counts = torch.zeros(self.n_clouds, dtype=torch.int64)
+            if spread == "one":
+                counts[0] = nq
+            else:
+                counts[:] = nq // self.n_clouds
+                counts[0] += nq - int(counts.sum())
+            qh = torch.cat([torch.zeros(1, dtype=torch.int64), torch.cumsum(counts, 0)])
+            d = self._dims(nq, qh, qh)
+            worst = max(worst, self._lib.aknn_forward_workspace_size(ctypes.byref(d)))
+        return worst
+
     # ------------------------------------------------------------------ clock
     def now(self) -> float:
         return time.perf_counter() - self.t0
 
     # ------------------------------------------------------------------ requests
     def submit(self, req: Request) -> None:
-        if not 0 <= req.cloud < len(self.clouds):
+        if not 0 <= req.cloud < self.n_clouds:
             raise ValueError(f"unknown cloud {req.cloud}")
         if req.q.dim() != 2 or req.q.shape[1] != 3 or req.q.shape[0] < 1:
             raise ValueError("request queries must have shape [Q, 3] with Q >= 1")
+        if req.q.dtype != self.dtype:
+            raise ValueError(f"request queries must be {self.dtype}")
         if req.q.shape[0] > self.max_batch_queries:
             raise ValueError(f"request of {req.q.shape[0]} queries exceeds max_batch_queries")
-        if self.batch_bytes(req.q.shape[0], req.cloud) > self.hbm_budget:
+        if self.batch_bytes(req.q.shape[0]) > self.hbm_budget:
             raise ValueError("request alone exceeds the HBM budget")
         self.waiting.append(req)
 
-    def batch_bytes(self, nq: int, cloud: int) -> int:
-        """HBM estimate of one batch: per-query outputs, gradients and search workspace,
-        plus the per-call point-side search structures of its cloud."""
-        P = self.clouds[cloud][0].shape[1]
-        el = self.clouds[cloud][0].element_size()
-        per_q = 3 * el + self.F * el * 3 + self.K * (2 * el + 8) + 3 * el + 64
-        return nq * per_q + P * (16 * el + 64) + (1 << 20)
+    def batch_bytes(self, nq: int) -> int:
+        """HBM estimate of one batch in flight: per-query inputs, outputs and search
+        workspace (the clouds and their index are resident and counted apart)."""
+        el = self.p.element_size()
+        per_q = 3 * el + self.F * el * 2 + self.K * (2 * el + 8) + 3 * el + 64
+        return nq * per_q + (1 << 20)
 
     # ------------------------------------------------------------------ engine
     def _launch(self, reqs: List[Request], slot: int, nbytes: int) -> None:
-        p, f, s, R = self.clouds[reqs[0].cloud]
-        st = self.streams[slot]
+        sl = self.slots[slot]
+        st = sl.stream
         t = self.now()
+        # queries grouped by cloud (arrival order within a cloud): offsets and rows
+        counts = [0] * self.n_clouds
         for r in reqs:
-            r.started = t
+            counts[r.cloud] += r.q.shape[0]
+        start = [0] * (self.n_clouds + 1)
+        for c in range(self.n_clouds):
+            start[c + 1] = start[c] + counts[c]
+        nq = start[-1]
+        sl.q_off_host.copy_(torch.tensor(start, dtype=torch.int64))
+        fill = start[:-1]
         with torch.cuda.stream(st):
-            qs = [r.q.to(self.device, non_blocking=True) for r in reqs]
-            q = (torch.cat(qs, 0) if len(qs) > 1 else qs[0])[None].contiguous()
-            tune = dict(self.tune or {})
-            tune["index"] = self.indexes[reqs[0].cloud]
-            f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, self.K, key=self.key,
-                                                                       tune=tune)
-            g = self.g_all[: q.shape[1]][None]
-            (dq,) = _C.knn_aggregate_aniso_backward(g, w_out, q, p, f, s, R, w_out, idxs, key=self.key,
-                                                   needs=(True, False, False, False, False))[:1]
-            f_host = torch.empty(f_out.shape[1:], dtype=f_out.dtype, pin_memory=True)
-            n_host = torch.empty(dq.shape[1:], dtype=dq.dtype, pin_memory=True)
-            f_host.copy_(f_out[0], non_blocking=True)
-            n_host.copy_(dq[0], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(st)
-        b = _Batch(reqs, ev, slot, nbytes, [q, f_out, w_out, dists, idxs, g, dq, f_host, n_host])
-        self.inflight.append(b)
+            sl.q_off.copy_(sl.q_off_host, non_blocking=True)
+            for r in reqs:
+                m = r.q.shape[0]
+                r.started = t
+                r.row = fill[r.cloud]
+                sl.q[r.row:r.row + m].copy_(r.q, non_blocking=True)
+                fill[r.cloud] += m
+        # forward and the query-side dq through the C ABI on the slot's stream, into the
+        # slot's buffers (inputs validated at construction and submission)
+        lib = self._lib
+        dims = self._dims(nq, sl.q_off, sl.q_off_host)
+        ws_bytes = lib.aknn_forward_workspace_size(ctypes.byref(dims))
+        if ws_bytes > sl.ws.numel():  # not expected: sized for the worst spread at start
+            with torch.cuda.stream(st):
+                sl.ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
+        hs = st.cuda_stream
+        _lib.check(lib.aknn_forward(ctypes.byref(dims), ctypes.byref(self.tune), sl.q.data_ptr(),
+                                    self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(), self.R.data_ptr(),
+                                    sl.f_out.data_ptr(), sl.w_out.data_ptr(), sl.dists.data_ptr(),
+                                    sl.idxs.data_ptr(), sl.ws.data_ptr(), ws_bytes, None, hs))
+        _lib.check(lib.aknn_backward(ctypes.byref(dims), _lib.NEED_DQ, self.g_all.data_ptr(), sl.q.data_ptr(),
+                                     self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(),
+                                     self.R.data_ptr(), sl.idxs.data_ptr(), None, None, None, None, 0,
+                                     sl.dq.data_ptr(), None, None, None, None, hs))
+        with torch.cuda.stream(st):
+            sl.f_host[:nq].copy_(sl.f_out[:nq], non_blocking=True)
+            sl.n_host[:nq].copy_(sl.dq[:nq], non_blocking=True)
+            sl.event.record(st)
+        self.inflight.append(_Batch(reqs, sl.event, slot, nbytes, nq))
         self.in_use += nbytes
         self.batches += 1
 
     def _form_batches(self) -> None:
         while self.free_slots and self.waiting:
-            head = self.waiting[0]
-            cloud = head.cloud
             take, nq = [], 0
-            for r in list(self.waiting):
-                if r.cloud != cloud:
-                    continue
+            for r in self.waiting:
                 n = r.q.shape[0]
                 if take and (nq + n > self.max_batch_queries or
-                             self.in_use + self.batch_bytes(nq + n, cloud) > self.hbm_budget):
+                             self.in_use + self.batch_bytes(nq + n) > self.hbm_budget):
                     break
                 take.append(r)
                 nq += n
-            nbytes = self.batch_bytes(nq, cloud)
+            nbytes = self.batch_bytes(nq)
             if self.in_use + nbytes > self.hbm_budget:
                 return  # wait for memory to free up
-            for r in take:
-                self.waiting.remove(r)
+            for _ in take:
+                self.waiting.popleft()
             self._launch(take, self.free_slots.pop(0), nbytes)
 
     def _reap(self) -> int:
@@ -166,16 +253,17 @@ class Server:
         for b in self.inflight:
             if b.event.query():
                 t = self.now()
-                f_host, n_host = b.outs[-2], b.outs[-1]
-                off = 0
+                sl = self.slots[b.slot]
                 for r in b.reqs:
                     m = r.q.shape[0]
-                    r.f_out = f_host[off:off + m]
-                    r.normal = n_host[off:off + m]
                     r.done = t
-                    off += m
-                    if self.on_done is not None:
+                    if self.on_done is not None:  # views, valid during the callback
+                        r.f_out = sl.f_host[r.row:r.row + m]
+                        r.normal = sl.n_host[r.row:r.row + m]
                         self.on_done(r)
+                    else:
+                        r.f_out = sl.f_host[r.row:r.row + m].clone()
+                        r.normal = sl.n_host[r.row:r.row + m].clone()
                 self.free_slots.append(b.slot)
                 self.in_use -= b.bytes
                 n += 1
@@ -191,8 +279,7 @@ class Server:
 
     def warm_up(self, repeats: int = 2) -> None:
         """Serve batches of every power-of-two size up to ``max_batch_queries`` on every
-        stream, so that the device and pinned-host caching allocators hold blocks of
-        each size class before timed serving starts."""
+        stream (first launches load code objects and size the workspaces)."""
         sizes = []
         n = 256
         while n < self.max_batch_queries:
@@ -200,14 +287,14 @@ class Server:
             n *= 2
         sizes.append(self.max_batch_queries)
         keep = self.on_done
-        self.on_done = lambda r: (setattr(r, "f_out", None), setattr(r, "normal", None))
+        self.on_done = lambda r: None
         try:
             for _ in range(repeats):
                 for n in sizes:
-                    for c in range(min(len(self.clouds), len(self.streams))):
-                        p = self.clouds[c][0][0]
-                        q = p[torch.arange(n, device=p.device) % p.shape[0]].cpu().pin_memory()
-                        self.submit(Request(c, q))
+                    for c in range(min(self.n_clouds, len(self.slots))):
+                        ps, pe = int(self.p_off_host[c]), int(self.p_off_host[c + 1])
+                        rows = torch.arange(n, device=self.device) % (pe - ps) + ps
+                        self.submit(Request(c, self.p[rows].cpu().pin_memory()))
                         self.poll()
                     self.drain()
         finally:

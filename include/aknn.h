@@ -92,6 +92,8 @@ typedef struct aknn_dims {
 #define AKNN_SEARCH_AUTO 0
 #define AKNN_SEARCH_BRUTE 1  /* scan every point of the cloud                       */
 #define AKNN_SEARCH_TILES 2  /* spatially sorted tiles, provably-useless tiles skipped */
+#define AKNN_SEARCH_WAVE 3   /* the same tiles walked per query (one wave each): for
+                                query sets sparse in their clouds; AUTO picks it there */
 
 /* Tunables of the neighbour search (0 = library default for the shape). */
 typedef struct aknn_tune {

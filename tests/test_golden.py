@@ -45,7 +45,7 @@ def test_oracle_reproduces_fixture(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("search", ["brute", "tiles"])
+@pytest.mark.parametrize("search", ["brute", "tiles", "wave"])
 @pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p))
 def test_gpu_matches_fixture(cuda, path, search):
     from cuda_knn_aggregate import _C

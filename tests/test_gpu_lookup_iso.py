@@ -85,7 +85,7 @@ def test_lookup_errors(cuda):
         _C.knn_lookup_aggregate_aniso_forward(b.q, b.p, b.f, b.sigma, b.R, bad, 4)
 
 
-@pytest.mark.parametrize("search", ["brute", "tiles"])
+@pytest.mark.parametrize("search", ["brute", "tiles", "wave"])
 @pytest.mark.parametrize("K", [1, 5, 16])
 def test_iso_forward_matches_oracle(cuda, search, K):
     b = workloads.clustered_batch(Q=2000, P=9000, K=K, F=12, seed=25)

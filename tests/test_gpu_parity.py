@@ -42,6 +42,12 @@ def assert_rel(name, got, ref, tol):
 
 
 # ----------------------------------------------------------------------------- neighbours
+def same_bits(a, b):
+    """Equal values, NaN where the other has NaN."""
+    a, b = a.cpu(), b.cpu()
+    return torch.equal(a.isnan(), b.isnan()) and torch.equal(a[~a.isnan()], b[~b.isnan()])
+
+
 def aniso_dists_at(b, idxs):
     """FP32 anisotropic keys of the given neighbours (the oracle's key expression)."""
     out = torch.empty(idxs.shape, dtype=b.q.dtype)
@@ -63,7 +69,7 @@ def test_knn_bitexact_readme(cuda, key):
     assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
 
 
-SEARCHES = ["brute", "tiles"]
+SEARCHES = ["brute", "tiles", "wave"]
 
 
 @pytest.mark.parametrize("search", SEARCHES)
@@ -87,11 +93,12 @@ def test_knn_bitexact_clustered_full_launch(cuda, search):
     assert torch.equal(dists.cpu()[:, rows], rd)
 
 
+@pytest.mark.parametrize("search", ["tiles", "wave"])
 @pytest.mark.parametrize("key", ["aniso", "euclidean"])
 @pytest.mark.parametrize("K", [1, 7, 16, 40, 64])
-def test_knn_tiles_k_and_key(cuda, K, key):
+def test_knn_tiles_k_and_key(cuda, K, key, search):
     b = workloads.clustered_batch(Q=3000, P=5000, K=K, F=4, seed=K)
-    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": "tiles"})
+    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": search})
     rd, ri = oracle_knn(b, key)
     assert torch.equal(idxs.cpu(), ri)
     assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
@@ -111,7 +118,8 @@ def test_knn_readme_literal_inputs(cuda, search):
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
-def test_knn_tiles_degenerate_scales(cuda):
+@pytest.mark.parametrize("search", ["tiles", "wave"])
+def test_knn_tiles_degenerate_scales(cuda, search):
     """Zero and negative scales (tiles that must always be scanned), far outliers and
     exact duplicates."""
     b = workloads.clustered_batch(Q=2000, P=3000, K=16, F=4, seed=5)
@@ -120,7 +128,7 @@ def test_knn_tiles_degenerate_scales(cuda):
     b.p[0, 900:905] += 50.0
     for t in (b.p, b.sigma, b.R):
         t[0, 2000:2400] = t[0, 1000:1400]
-    _, _, dists, idxs = fwd(b, cuda, tune={"search": "tiles"})
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": search})
     rd, ri = oracle_knn(b)
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
@@ -142,6 +150,64 @@ def test_knn_tiles_split_walk(cuda, ways, key):
     rd, ri = oracle_knn(b, key)
     assert torch.equal(idxs.cpu(), ri)
     assert torch.equal(dists.cpu(), aniso_dists_at(b, ri))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("key", ["aniso", "euclidean"])
+@pytest.mark.parametrize("search", ["auto", "wave"])
+def test_knn_wave_sparse_queries(cuda, search, key, dtype):
+    """Few queries spread over a cloud (the per-query walk's case, which AUTO picks
+    here), with always-scanned tiles, exact duplicates and a NaN query; same bits as the
+    oracle, in both precisions."""
+    b = workloads.clustered_batch(Q=700, P=30000, K=16, F=4, seed=31)
+    b = workloads.Batch(*(t.to(dtype) for t in (b.q, b.p, b.f, b.sigma, b.R)), b.K)
+    g = torch.Generator().manual_seed(32)
+    b.q[0] = torch.rand(700, 3, generator=g) * 1.2 - 0.1
+    b.q[0, 5] = float("nan")
+    b.sigma[0, 100:140, 1] = 0.0
+    b.sigma[0, 400:420, 2] = -0.01
+    for t in (b.p, b.sigma, b.R):
+        t[0, 20000:20400] = t[0, 1000:1400]
+    _, _, dists, idxs = fwd(b, cuda, key=key, tune={"search": search} if search != "auto" else None)
+    rd, ri = oracle_knn(b, key)
+    assert torch.equal(idxs.cpu(), ri)
+    assert same_bits(dists, aniso_dists_at(b, ri))
+
+
+@pytest.mark.parametrize("search", SEARCHES)
+def test_knn_rows_without_k_comparable_keys(cuda, search):
+    """A NaN query compares with no point: like the reference's scan it keeps points
+    0..K-1, with NaN keys.  A cloud with fewer than K points whose keys are not NaN:
+    those come first in (key, index) order and the remaining slots take the lowest
+    unused indices.  Every index stays inside its cloud (the aggregation gathers by
+    them)."""
+    b = workloads.clustered_batch(Q=300, P=9000, K=16, F=4, seed=51)
+    b.q[0, [0, 77, 299]] = float("nan")
+    f_out, _, dists, idxs = fwd(b, cuda, tune={"search": search})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and same_bits(dists, rd)
+    assert torch.equal(idxs.cpu()[0, 77], torch.arange(16)) and bool(f_out[0, 77].isnan().all())
+    c = workloads.uniform_batch(1, 50, 40, 16, 4, seed=52)
+    c.p[0, 5:35] = float("nan")                      # 10 points with comparable keys
+    _, _, dists, idxs = fwd(c, cuda, tune={"search": search})
+    idxs, dists = idxs.cpu()[0], dists.cpu()[0]
+    finite = torch.tensor(list(range(5)) + list(range(35, 40)))
+    for r in range(50):
+        assert set(idxs[r, :10].tolist()) == set(finite.tolist())
+        assert torch.equal(idxs[r, 10:], torch.tensor([5, 6, 7, 8, 9, 10]))
+        assert bool(dists[r, 10:].isnan().all()) and not bool(dists[r, :10].isnan().any())
+
+
+@pytest.mark.parametrize("shape", ["clustered", "uniform2"])
+def test_knn_wave_large_cloud(cuda, shape):
+    """Clouds of more than 64 super-tiles (the per-query walk orders them in chunks)."""
+    if shape == "clustered":
+        b = workloads.clustered_batch(Q=300, P=150000, K=16, F=4, seed=41)
+    else:
+        b = workloads.uniform_batch(2, 200, 140000, 16, 4, seed=42)
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": "wave"})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
 @pytest.mark.parametrize("qpt", [1, 2, 4])

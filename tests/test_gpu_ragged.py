@@ -31,7 +31,7 @@ def small_ragged(seed=0, with_empty=False):
     return r
 
 
-@pytest.mark.parametrize("search", ["brute", "tiles"])
+@pytest.mark.parametrize("search", ["brute", "tiles", "wave"])
 @pytest.mark.parametrize("with_empty", [False, True])
 def test_ragged_forward_equals_per_cloud(cuda, search, with_empty):
     r = small_ragged(1, with_empty)
