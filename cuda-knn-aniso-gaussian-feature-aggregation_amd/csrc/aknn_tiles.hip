@@ -31,6 +31,7 @@ namespace aknn {
 constexpr int kTile = 64;
 constexpr int kSuper = 16;  // tiles per super-tile
 constexpr int kTileSentinel = kNoPoint;
+constexpr int kWaveSerialInserts = 8;  // per-query walk: more candidates in a tile -> sort
 
 // Workspace pieces of the tiled search (see carve()).
 struct PrunedState {
@@ -807,15 +808,56 @@ __device__ __forceinline__ T lane_value(T v, int l) {
   }
 }
 
+// DPP move: lanes whose source is outside the pattern, or that the row / bank masks
+// leave out, keep `old`.
+template <int CTRL, int ROW_MASK, int BANK_MASK, typename T>
+__device__ __forceinline__ T dpp_move(T old, T v) {
+  if constexpr (sizeof(T) == 4) {
+    int o, x;
+    __builtin_memcpy(&o, &old, 4);
+    __builtin_memcpy(&x, &v, 4);
+    const int r = __builtin_amdgcn_update_dpp(o, x, CTRL, ROW_MASK, BANK_MASK, false);
+    T out;
+    __builtin_memcpy(&out, &r, 4);
+    return out;
+  } else {
+    int o[2], x[2], r[2];
+    __builtin_memcpy(o, &old, 8);
+    __builtin_memcpy(x, &v, 8);
+    r[0] = __builtin_amdgcn_update_dpp(o[0], x[0], CTRL, ROW_MASK, BANK_MASK, false);
+    r[1] = __builtin_amdgcn_update_dpp(o[1], x[1], CTRL, ROW_MASK, BANK_MASK, false);
+    T out;
+    __builtin_memcpy(&out, r, 8);
+    return out;
+  }
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr3 = 0x113, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// Minimum over the wave of NaN-free values (`inf`: +infinity held in a register), with
+// DPP row shifts and broadcasts: lane 63 ends with the minimum.
+template <typename T>
+__device__ __forceinline__ T wave_min(T x, T inf) {
+  auto mn = [](T a, T b) { return b < a ? b : a; };
+  T a = mn(x, dpp_move<kDppRowShr1, 0xf, 0xf>(inf, x));
+  a = mn(a, dpp_move<kDppRowShr2, 0xf, 0xf>(inf, x));
+  a = mn(a, dpp_move<kDppRowShr3, 0xf, 0xf>(inf, x));
+  a = mn(a, dpp_move<kDppRowShr4, 0xf, 0xe>(inf, a));
+  a = mn(a, dpp_move<kDppRowShr8, 0xf, 0xc>(inf, a));
+  a = mn(a, dpp_move<kDppRowBcast15, 0xa, 0xf>(inf, a));
+  a = mn(a, dpp_move<kDppRowBcast31, 0xc, 0xf>(inf, a));
+  return lane_value<T>(a, 63);
+}
+
 // Lowest lane among `sel` holding the smallest `v` (NaN ordered first); some lane must
 // be selected.
 template <typename T>
-__device__ __forceinline__ int wave_argmin(T v, bool sel) {
-  const T x = sel ? (v != v ? -__builtin_inf() : v) : __builtin_inf();
-  T m = x;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
-  return __builtin_ctzll(__ballot(sel && x == m));
+__device__ __forceinline__ int wave_argmin(T v, bool sel, T inf) {
+  const T x = sel ? (v != v ? -inf : v) : inf;
+  const T m = wave_min<T>(x, inf);
+  unsigned long long hit = __ballot(sel && x == m);
+  if (hit == 0) hit = __ballot(sel);  // never taken when the minimum is exact; stays in range
+  return __builtin_ctzll(hit);
 }
 
 // Certified lower bound of a box's keys for the query (negative infinity: "always
@@ -831,6 +873,22 @@ __device__ __forceinline__ T box_bound(const float* __restrict__ h, T qx, T qy, 
   return mu < T(0) ? -__builtin_inf() : mu * (dx * dx + dy * dy + dz * dz);
 }
 
+template <typename T>
+__device__ __forceinline__ bool lex_less(T a, int ia, T b, int ib) {
+  return (a < b) | ((a == b) & (ia < ib));
+}
+
+// Compare-exchange with lane ^ j in the (key, index) order: the lane keeps the smaller
+// of the pair if keep_min, else the larger.
+template <typename T>
+__device__ __forceinline__ void cmp_exchange(T& k, int& i, int j, bool keep_min) {
+  const T ok = __shfl_xor(k, j, 64);
+  const int oi = __shfl_xor(i, j, 64);
+  const bool take = keep_min ? lex_less(ok, oi, k, i) : lex_less(k, i, ok, oi);
+  k = take ? ok : k;
+  i = take ? oi : i;
+}
+
 template <typename T, int KEY>
 __global__ __launch_bounds__(256) void knn_wave_kernel(
     const T* __restrict__ q, Clouds cl, int64_t qrows, const int64_t* __restrict__ ptiles,
@@ -838,7 +896,9 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
     const float* __restrict__ shdr, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
     unsigned long long* __restrict__ stats) {
   const int lane = threadIdx.x & 63;
-  const int64_t row = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // the wave's query (made wave-uniform, so the walk's bookkeeping is scalar)
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) +
+                      __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   if (row >= qrows) return;  // whole waves
   const int64_t b = cl.cloud_of_query(row);
   int64_t tb, te, sb, se;
@@ -859,7 +919,11 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
   // the encoding keeps the low half (+inf became 0.0)
   T inf_v = __builtin_inf();
   asm volatile("" : "+v"(inf_v));
-  T lk = inf_v;  // lanes 0..K-1: the sorted list
+  // lanes 0..Kp-1 (K rounded up to a power of two): the sorted list; lane K-1 holds
+  // the pruning bound, lanes K..Kp-1 pad the bitonic merge
+  int Kp = 1;
+  while (Kp < K) Kp <<= 1;
+  T lk = inf_v;
   int li = kTileSentinel;
   T last_k = lane_value<T>(inf_v, 0);
   int last_i = kTileSentinel;
@@ -876,7 +940,7 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
     for (int64_t c = 0; c < nchunk; ++c) {
       const int64_t s = sb + c * 64 + lane;
       const T v = s < se ? box_bound<T>(shdr + s * 8, qx, qy, qz) : __builtin_inf();
-      const int l = wave_argmin<T>(v, s < se);
+      const int l = wave_argmin<T>(v, s < se, inf_v);
       const T m = lane_value<T>(v, l);
       if (m < best) {
         best = m;
@@ -885,54 +949,81 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
     }
     first = bc;
   }
+  // One tile's 64 records against the list (lane j: record j).
+  auto consume = [&](const Rec16<T>& r) {
+    const T ck = rec_key<T, KEY>(qx, qy, qz, r);
+    const int ci = rec_index(r.v[15]);
+    const bool cand = lex_less(ck, ci, last_k, last_i);
+    unsigned long long m = __ballot(cand);
+    if (m) ++n_scan;
+    if (__builtin_popcountll(m) > kWaveSerialInserts) {
+      // many candidates (the first tiles): sort the tile's 64 (key, index) pairs
+      // across the wave, then merge the K smallest into the list (the elementwise
+      // minimum of the list and the reversed candidates is bitonic and holds the Kp
+      // smallest of both; a bitonic merge orders it)
+      T sk = cand ? ck : inf_v;
+      int si = cand ? ci : kTileSentinel;
+#pragma unroll
+      for (int kk = 2; kk <= 64; kk <<= 1)
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) cmp_exchange(sk, si, j, ((lane & j) == 0) == ((lane & kk) == 0));
+      const T rk = __shfl(sk, (Kp - 1 - lane) & 63, 64);
+      const int ri = __shfl(si, (Kp - 1 - lane) & 63, 64);
+      if (lane < Kp && lex_less(rk, ri, lk, li)) {
+        lk = rk;
+        li = ri;
+      }
+      for (int j = Kp >> 1; j > 0; j >>= 1) cmp_exchange(lk, li, j, (lane & j) == 0);
+      last_k = lane_value<T>(lk, K - 1);
+      last_i = __builtin_amdgcn_readlane(li, K - 1);
+      m = 0;
+    }
+    while (m) {
+      const int l = __builtin_ctzll(m);
+      m &= m - 1;
+      const T k = lane_value<T>(ck, l);
+      const int i = __builtin_amdgcn_readlane(ci, l);
+      if (!lex_less(k, i, last_k, last_i)) continue;
+      // insert into the sorted list: lanes at or after the insertion point shift up
+      const bool lt = (lane < Kp) & lex_less(k, i, lk, li);
+      const T pk = dpp_move<kDppWaveShr1, 0xf, 0xf>(lk, lk);
+      const int pi = dpp_move<kDppWaveShr1, 0xf, 0xf>(li, li);
+      const bool plt = dpp_move<kDppWaveShr1, 0xf, 0xf>(0, static_cast<int>(lt)) != 0;
+      if (lt) {
+        lk = plt ? pk : k;
+        li = plt ? pi : i;
+      }
+      last_k = lane_value<T>(lk, K - 1);
+      last_i = __builtin_amdgcn_readlane(li, K - 1);
+    }
+  };
+
   int64_t up = first, down = first - 1;
   for (int64_t step = 0; step < nchunk; ++step) {
     const bool take_up = (up < nchunk) && ((step & 1) == 0 || down < 0);
     const int64_t c = take_up ? up++ : down--;
     const int64_t s = sb + c * 64 + lane;
     const bool sin = s < se;
-    const T slb = sin ? box_bound<T>(shdr + s * 8, qx, qy, qz) : __builtin_inf();
+    const T slb = sin ? box_bound<T>(shdr + s * 8, qx, qy, qz) : inf_v;
     bool sdone = !sin;
     while (true) {
       const bool ssel = !sdone && !(slb > last_k);
       if (!__any(ssel)) break;
-      const int sl = wave_argmin<T>(slb, ssel);
+      const int sl = wave_argmin<T>(slb, ssel, inf_v);
       if (lane == sl) sdone = true;
       ++n_super;
       const int64_t t0 = tb + (c * 64 + sl) * kSuper;
       const int64_t nt = te - t0 < kSuper ? te - t0 : kSuper;
       const bool tin = lane < nt;
-      const T tlb = tin ? box_bound<T>(hdr + (t0 + lane) * 8, qx, qy, qz) : __builtin_inf();
+      const T tlb = tin ? box_bound<T>(hdr + (t0 + lane) * 8, qx, qy, qz) : inf_v;
       bool tdone = !tin;
       while (true) {
         const bool tsel = !tdone && !(tlb > last_k);
         if (!__any(tsel)) break;
-        const int tl = wave_argmin<T>(tlb, tsel);
+        const int tl = wave_argmin<T>(tlb, tsel, inf_v);
         if (lane == tl) tdone = true;
         ++n_tile;
-        const Rec16<T> r = load_rec16(rec + (t0 + tl) * (kTile * 16) + lane * 16);
-        const T ck = rec_key<T, KEY>(qx, qy, qz, r);
-        const int ci = rec_index(r.v[15]);
-        unsigned long long m = __ballot((ck < last_k) | ((ck == last_k) & (ci < last_i)));
-        if (m) ++n_scan;
-        while (m) {
-          const int l = __builtin_ctzll(m);
-          m &= m - 1;
-          const T k = lane_value<T>(ck, l);
-          const int i = __builtin_amdgcn_readlane(ci, l);
-          if (!((k < last_k) | ((k == last_k) & (i < last_i)))) continue;
-          // insert into the sorted list: lanes at or after the insertion point shift up
-          const bool lt = (lane < K) & ((k < lk) | ((k == lk) & (i < li)));
-          const T pk = __shfl_up(lk, 1, 64);
-          const int pi = __shfl_up(li, 1, 64);
-          const bool plt = (lane > 0) & (__shfl_up(static_cast<int>(lt), 1, 64) != 0);
-          if (lt) {
-            lk = plt ? pk : k;
-            li = plt ? pi : i;
-          }
-          last_k = lane_value<T>(lk, K - 1);
-          last_i = __builtin_amdgcn_readlane(li, K - 1);
-        }
+        consume(load_rec16(rec + (t0 + tl) * (kTile * 16) + lane * 16));
       }
     }
   }
