@@ -213,6 +213,7 @@ def ragged_bench(args):
 def serving_bench(args):
     """Resident clouds, Poisson arrivals at several rates, continuous batching; p50/p99
     latency (arrival to results on the host) and served queries/s per rate."""
+    import gc
     import math
 
     import torch
@@ -251,11 +252,16 @@ def serving_bench(args):
     server.warm_up()                            # code objects, device and pinned-host allocator caches
     server.replay(requests(100, 500.0, 1))
     rows = []
-    for rate in (250.0, 500.0, 1000.0, 2000.0, 3000.0, 4000.0, 6000.0, 8000.0):
+    for rate in (250.0, 500.0, 1000.0, 2000.0, 3000.0, 4000.0, 5000.0, 6000.0, 8000.0):
         n = int(min(2000, max(200, rate * 2.0)))
         reqs = requests(n, rate, int(rate))
         b0 = server.batches
-        server.replay(reqs)
+        gc.collect()          # no collector pause inside the served interval
+        gc.disable()
+        try:
+            server.replay(reqs)
+        finally:
+            gc.enable()
         st = latency_stats(reqs)
         st.update(rate_req_per_s=rate, batches=server.batches - b0,
                   offered_queries_per_s=rate * sum(r.q.shape[0] for r in reqs) / len(reqs))
