@@ -516,11 +516,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
   const int iway = static_cast<int>(way);
 
   // Split walk (ways > 1): this block owns the tiles t with (t - tb) % ways == way and
-  // prunes with thr = the smallest last-kept key any way has published for the query.
+  // prunes, and filters insertions, with thr = the smallest last-kept key any way has
+  // published for the query (starting from the query's seed bound).
   // Every way's list is a subset's top-KMAX, so its last key bounds the full top-KMAX
   // from above: a tile skipped against thr holds no point of the final result, whatever
   // the timing of the exchange.
-  T thr = __builtin_inf();
+  T thr = (ways > 1 && valid) ? Ord::dec(bnd[pos]) : __builtin_inf();
   unsigned n_super = 0, n_tile = 0, n_scan = 0, n_pts = 0;
   Outward so(sb, se, seed_s);
   for (int sstep = 0; sstep < se - sb; ++sstep) {
@@ -572,8 +573,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         const Rec16<T> b1 = load_rec16(nx + 16);
         const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
         const T c1 = rec_key<T, KEY>(qx, qy, qz, a1);
-        if (c0 <= key[KMAX - 1]) bubble_insert<KMAX>(key, id, c0, rec_index(a0.v[15]));
-        if (c1 <= key[KMAX - 1]) bubble_insert<KMAX>(key, id, c1, rec_index(a1.v[15]));
+        // a key above the shared bound is above the final K-th key: never inserted
+        if (c0 <= fmin(key[KMAX - 1], thr)) bubble_insert<KMAX>(key, id, c0, rec_index(a0.v[15]));
+        if (c1 <= fmin(key[KMAX - 1], thr)) bubble_insert<KMAX>(key, id, c1, rec_index(a1.v[15]));
         a0 = b0;
         a1 = b1;
         u = un;
