@@ -31,7 +31,7 @@ namespace aknn {
 constexpr int kTile = 64;
 constexpr int kSuper = 16;  // tiles per super-tile
 constexpr int kTileSentinel = kNoPoint;
-constexpr int kWaveSerialInserts = 8;  // per-query walk: more candidates in a tile -> sort
+constexpr int kWaveSerialInserts = 16;  // per-query walk: more candidates in a tile -> sort
 
 // Workspace pieces of the tiled search (see carve()).
 struct PrunedState {
