@@ -584,30 +584,52 @@ hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
 
 // ------------------------------------------------------------------ dq from contributions
 // dq[q] = sum over k of contrib[q*K + k] in slot order (fixed association).
-template <typename T>
-__global__ void contrib_reduce_kernel(const T* __restrict__ contrib, int64_t rows, int K,
-                                      T* __restrict__ dq) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= rows * 3) return;
-  const int64_t q = i / 3;
-  const int c = static_cast<int>(i % 3);
-  const T* src = contrib + q * K * 3 + c;
-  T s = T(0);
-  for (int k = 0; k < K; ++k) s += src[k * 3];
-  dq[i] = s;
+// dq[q] = sum over k of entry (q, k)'s share, in slot order: one thread per query.  When
+// a query's 3K shares come in whole 16-byte vectors (FP32, K a multiple of 4) the
+// thread reads them as such, otherwise one value at a time.
+template <typename T, bool VEC4>
+__global__ __launch_bounds__(256) void contrib_reduce_kernel(const T* __restrict__ contrib, int64_t rows, int K,
+                                                             T* __restrict__ dq) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= rows) return;
+  const T* src = contrib + q * K * 3;
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  if constexpr (VEC4) {
+    const float4* v = reinterpret_cast<const float4*>(src);
+    for (int k = 0; k < K; k += 4) {  // four entries = three vectors
+      const float4 a = v[0], b = v[1], c = v[2];
+      v += 3;
+      s0 += a.x; s1 += a.y; s2 += a.z;
+      s0 += a.w; s1 += b.x; s2 += b.y;
+      s0 += b.z; s1 += b.w; s2 += c.x;
+      s0 += c.y; s1 += c.z; s2 += c.w;
+    }
+  } else {
+    for (int k = 0; k < K; ++k) {
+      s0 += src[k * 3 + 0];
+      s1 += src[k * 3 + 1];
+      s2 += src[k * 3 + 2];
+    }
+  }
+  dq[q * 3 + 0] = s0;
+  dq[q * 3 + 1] = s1;
+  dq[q * 3 + 2] = s2;
 }
 
 hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, void* dq, int32_t dtype,
                                  hipStream_t st) {
   if (rows == 0) return hipSuccess;
-  const int64_t n = rows * 3;
-  const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
-  if (dtype == AKNN_F32)
-    hipLaunchKernelGGL(contrib_reduce_kernel<float>, grid, block, 0, st, static_cast<const float*>(contrib),
-                       rows, static_cast<int>(K), static_cast<float*>(dq));
+  const dim3 grid(static_cast<unsigned>((rows + 255) / 256)), block(256);
+  const bool vec4 = dtype == AKNN_F32 && K % 4 == 0 && (reinterpret_cast<uintptr_t>(contrib) % 16) == 0;
+  if (dtype == AKNN_F32 && vec4)
+    hipLaunchKernelGGL((contrib_reduce_kernel<float, true>), grid, block, 0, st,
+                       static_cast<const float*>(contrib), rows, static_cast<int>(K), static_cast<float*>(dq));
+  else if (dtype == AKNN_F32)
+    hipLaunchKernelGGL((contrib_reduce_kernel<float, false>), grid, block, 0, st,
+                       static_cast<const float*>(contrib), rows, static_cast<int>(K), static_cast<float*>(dq));
   else
-    hipLaunchKernelGGL(contrib_reduce_kernel<double>, grid, block, 0, st, static_cast<const double*>(contrib),
-                       rows, static_cast<int>(K), static_cast<double*>(dq));
+    hipLaunchKernelGGL((contrib_reduce_kernel<double, false>), grid, block, 0, st,
+                       static_cast<const double*>(contrib), rows, static_cast<int>(K), static_cast<double*>(dq));
   return hipGetLastError();
 }
 
