@@ -210,6 +210,19 @@ def test_knn_wave_large_cloud(cuda, shape):
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
+@pytest.mark.parametrize("K", [1, 5, 12, 40])
+@pytest.mark.parametrize("ways", [4, 64])
+def test_knn_tiles_split_walk_k_values(cuda, ways, K):
+    """Split walk with K below the kernel's list length (the shared bound and the seed
+    are K-th keys, the lists keep more): same bits as the oracle."""
+    b = workloads.clustered_batch(Q=400, P=30000, K=K, F=4, seed=60 + K)
+    g = torch.Generator().manual_seed(61)
+    b.q[0] = torch.rand(400, 3, generator=g) * 1.2 - 0.1
+    _, _, dists, idxs = fwd(b, cuda, tune={"search": "tiles", "point_split": ways})
+    rd, ri = oracle_knn(b)
+    assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
+
+
 @pytest.mark.parametrize("qpt", [1, 2, 4])
 @pytest.mark.parametrize("split", [1, 2, 4, 8])
 def test_knn_launch_shapes(cuda, qpt, split):
