@@ -115,7 +115,39 @@ __global__ __launch_bounds__(256) void scan_down_kernel(uint32_t* __restrict__ x
   }
 }
 
+// Small scans (the sorts of up to 128K keys): one block of 1024 threads, each thread
+// owning a run of consecutive elements -- one launch instead of three.
+constexpr int kScanOneThreads = 1024;
+constexpr int64_t kScanOneMax = 8192;
+
+__global__ __launch_bounds__(kScanOneThreads) void scan_one_block_kernel(uint32_t* __restrict__ x, int64_t n) {
+  __shared__ uint32_t s[kScanOneThreads];
+  const int64_t per = (n + kScanOneThreads - 1) / kScanOneThreads;
+  const int64_t b = threadIdx.x * per;
+  const int64_t e = b + per < n ? b + per : n;
+  uint32_t local = 0;
+  for (int64_t j = b; j < e; ++j) local += x[j];
+  s[threadIdx.x] = local;
+  __syncthreads();
+  for (int off = 1; off < kScanOneThreads; off <<= 1) {  // inclusive Hillis-Steele
+    const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - local;
+  for (int64_t j = b; j < e; ++j) {
+    const uint32_t v = x[j];
+    x[j] = run;
+    run += v;
+  }
+}
+
 static hipError_t exclusive_scan(uint32_t* x, int64_t n, uint32_t* sums, hipStream_t st) {
+  if (n <= kScanOneMax) {
+    hipLaunchKernelGGL(scan_one_block_kernel, dim3(1), dim3(kScanOneThreads), 0, st, x, n);
+    return hipGetLastError();
+  }
   const int64_t chunks = (n + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, st, x, n, sums);
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, st, sums, chunks);
