@@ -121,21 +121,33 @@ constexpr int kScanOneThreads = 1024;
 constexpr int64_t kScanOneMax = 8192;
 
 __global__ __launch_bounds__(kScanOneThreads) void scan_one_block_kernel(uint32_t* __restrict__ x, int64_t n) {
-  __shared__ uint32_t s[kScanOneThreads];
+  __shared__ uint32_t wsum[kScanOneThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t per = (n + kScanOneThreads - 1) / kScanOneThreads;
   const int64_t b = threadIdx.x * per;
   const int64_t e = b + per < n ? b + per : n;
   uint32_t local = 0;
   for (int64_t j = b; j < e; ++j) local += x[j];
-  s[threadIdx.x] = local;
-  __syncthreads();
-  for (int off = 1; off < kScanOneThreads; off <<= 1) {  // inclusive Hillis-Steele
-    const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
+  // inclusive scan within the wave, then over the wave totals
+  uint32_t incl = local;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
   }
-  uint32_t run = s[threadIdx.x] - local;
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  if (wv == 0) {
+    uint32_t w = lane < kScanOneThreads / 64 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int off = 1; off < kScanOneThreads / 64; off <<= 1) {
+      const uint32_t t = __shfl_up(w, off, 64);
+      if (lane >= off) w += t;
+    }
+    if (lane < kScanOneThreads / 64) wsum[lane] = w;  // inclusive over waves
+  }
+  __syncthreads();
+  uint32_t run = (wv > 0 ? wsum[wv - 1] : 0u) + incl - local;
   for (int64_t j = b; j < e; ++j) {
     const uint32_t v = x[j];
     x[j] = run;
