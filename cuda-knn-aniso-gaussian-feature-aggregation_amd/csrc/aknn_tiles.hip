@@ -880,15 +880,75 @@ __device__ __forceinline__ bool lex_less(T a, int ia, T b, int ib) {
   return (a < b) | ((a == b) & (ia < ib));
 }
 
-// Compare-exchange with lane ^ j in the (key, index) order: the lane keeps the smaller
+// Value of lane ^ J: DPP quad permutes (J = 1, 2) and row shifts (J = 4, 8), a
+// ds_swizzle (J = 16), a permute through LDS (J = 32).
+constexpr int kDppQuadSwap1 = 0xB1, kDppQuadSwap2 = 0x4E, kDppRowShl4 = 0x104, kDppRowShl8 = 0x108;
+constexpr int kSwizzleXor16 = 0x401F;  // bitmask mode: and 0x1f, xor 0x10
+
+template <int J, typename T>
+__device__ __forceinline__ T lane_xor(T v, int lane) {
+  if constexpr (J == 1) {
+    return dpp_move<kDppQuadSwap1, 0xf, 0xf>(v, v);
+  } else if constexpr (J == 2) {
+    return dpp_move<kDppQuadSwap2, 0xf, 0xf>(v, v);
+  } else if constexpr (J == 4) {
+    const T up = dpp_move<kDppRowShl4, 0xf, 0xf>(v, v), dn = dpp_move<kDppRowShr4, 0xf, 0xf>(v, v);
+    return (lane & 4) ? dn : up;
+  } else if constexpr (J == 8) {
+    const T up = dpp_move<kDppRowShl8, 0xf, 0xf>(v, v), dn = dpp_move<kDppRowShr8, 0xf, 0xf>(v, v);
+    return (lane & 8) ? dn : up;
+  } else if constexpr (J == 16) {
+    if constexpr (sizeof(T) == 4) {
+      int x;
+      __builtin_memcpy(&x, &v, 4);
+      x = __builtin_amdgcn_ds_swizzle(x, kSwizzleXor16);
+      T out;
+      __builtin_memcpy(&out, &x, 4);
+      return out;
+    } else {
+      int x[2];
+      __builtin_memcpy(x, &v, 8);
+      x[0] = __builtin_amdgcn_ds_swizzle(x[0], kSwizzleXor16);
+      x[1] = __builtin_amdgcn_ds_swizzle(x[1], kSwizzleXor16);
+      T out;
+      __builtin_memcpy(&out, x, 8);
+      return out;
+    }
+  } else {
+    static_assert(J == 32, "lane_xor: J in 1..32");
+    return __shfl_xor(v, 32, 64);
+  }
+}
+
+// Compare-exchange with lane ^ J in the (key, index) order: the lane keeps the smaller
 // of the pair if keep_min, else the larger.
-template <typename T>
-__device__ __forceinline__ void cmp_exchange(T& k, int& i, int j, bool keep_min) {
-  const T ok = __shfl_xor(k, j, 64);
-  const int oi = __shfl_xor(i, j, 64);
+template <int J, typename T>
+__device__ __forceinline__ void cmp_exchange(T& k, int& i, int lane, bool keep_min) {
+  const T ok = lane_xor<J>(k, lane);
+  const int oi = lane_xor<J>(i, lane);
   const bool take = keep_min ? lex_less(ok, oi, k, i) : lex_less(k, i, ok, oi);
   k = take ? ok : k;
   i = take ? oi : i;
+}
+
+// Bitonic sort of the wave's 64 (key, index) pairs, ascending by lane: stage KK, steps
+// J = KK/2 .. 1.
+template <int KK, int J, typename T>
+__device__ __forceinline__ void bitonic_steps(T& k, int& i, int lane) {
+  cmp_exchange<J>(k, i, lane, ((lane & J) == 0) == ((lane & KK) == 0));
+  if constexpr (J > 1) bitonic_steps<KK, J / 2>(k, i, lane);
+}
+template <int KK, typename T>
+__device__ __forceinline__ void bitonic_sort64(T& k, int& i, int lane) {
+  bitonic_steps<KK, KK / 2>(k, i, lane);
+  if constexpr (KK < 64) bitonic_sort64<KK * 2>(k, i, lane);
+}
+
+// Ascending half-cleaner steps J .. 1 (orders a bitonic run of 2J lanes).
+template <int J, typename T>
+__device__ __forceinline__ void bitonic_merge(T& k, int& i, int lane) {
+  cmp_exchange<J>(k, i, lane, (lane & J) == 0);
+  if constexpr (J > 1) bitonic_merge<J / 2>(k, i, lane);
 }
 
 template <typename T, int KEY>
@@ -965,17 +1025,22 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
       // smallest of both; a bitonic merge orders it)
       T sk = cand ? ck : inf_v;
       int si = cand ? ci : kTileSentinel;
-#pragma unroll
-      for (int kk = 2; kk <= 64; kk <<= 1)
-#pragma unroll
-        for (int j = kk >> 1; j > 0; j >>= 1) cmp_exchange(sk, si, j, ((lane & j) == 0) == ((lane & kk) == 0));
+      bitonic_sort64<2>(sk, si, lane);
       const T rk = __shfl(sk, (Kp - 1 - lane) & 63, 64);
       const int ri = __shfl(si, (Kp - 1 - lane) & 63, 64);
       if (lane < Kp && lex_less(rk, ri, lk, li)) {
         lk = rk;
         li = ri;
       }
-      for (int j = Kp >> 1; j > 0; j >>= 1) cmp_exchange(lk, li, j, (lane & j) == 0);
+      switch (Kp) {
+        case 64: bitonic_merge<32>(lk, li, lane); break;
+        case 32: bitonic_merge<16>(lk, li, lane); break;
+        case 16: bitonic_merge<8>(lk, li, lane); break;
+        case 8: bitonic_merge<4>(lk, li, lane); break;
+        case 4: bitonic_merge<2>(lk, li, lane); break;
+        case 2: bitonic_merge<1>(lk, li, lane); break;
+        default: break;
+      }
       last_k = lane_value<T>(lk, K - 1);
       last_i = __builtin_amdgcn_readlane(li, K - 1);
       m = 0;
