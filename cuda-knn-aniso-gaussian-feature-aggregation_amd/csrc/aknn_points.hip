@@ -520,7 +520,10 @@ static int pick_vec_points(const PointLaunch& L) {
   auto aligned = [&](const void* ptr, int v) {
     return ptr == nullptr || (reinterpret_cast<uintptr_t>(ptr) % (sizeof(T) * v)) == 0;
   };
-  for (int v = maxv; v > 1; v >>= 1)
+  // no wider than one chunk needs: 16 lanes x v >= F with v as small as that allows
+  int v0 = 1;
+  while (v0 < maxv && 16 * v0 < L.F) v0 <<= 1;
+  for (int v = v0; v > 1; v >>= 1)
     if (L.F % v == 0 && aligned(L.f, v) && aligned(L.df, v) && aligned(L.grad_f_out, v)) return v;
   return 1;
 }
