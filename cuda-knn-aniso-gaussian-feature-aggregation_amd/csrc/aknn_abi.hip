@@ -51,13 +51,12 @@ class Span {
     rec_.a = pool_event();
     rec_.b = pool_event();
     if (rec_.a && rec_.b) {
-      hipEventRecord(rec_.a, st_);
-      on_ = true;
+      on_ = hipEventRecord(rec_.a, st_) == hipSuccess;  // tracing only: a failed record drops the span
     }
   }
   ~Span() {
     if (!on_) return;
-    hipEventRecord(rec_.b, st_);
+    if (hipEventRecord(rec_.b, st_) != hipSuccess) return;
     std::lock_guard<std::mutex> lk(g_trace_mu);
     g_trace.push_back(rec_);
   }
@@ -361,7 +360,7 @@ int aknn_trace_collect(char* names, size_t names_bytes, float* ms, int max_recor
   size_t used = 0;
   for (const TraceRec& r : recs) {
     float t = -1.f;
-    if (hipEventSynchronize(r.b) == hipSuccess) hipEventElapsedTime(&t, r.a, r.b);
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) t = -1.f;
     if (n < max_records && ms && names) {
       const size_t len = strlen(r.name) + 1;
       if (used + len <= names_bytes) {

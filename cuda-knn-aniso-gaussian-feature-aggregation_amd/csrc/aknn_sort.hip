@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void scan_reduce_kernel(const uint32_t* __rest
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
@@ -66,7 +66,7 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t* s, uint32_t v, uin
   s[threadIdx.x] = v;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    const uint32_t t = static_cast<int>(threadIdx.x) >= off ? s[threadIdx.x - off] : 0u;
     __syncthreads();
     s[threadIdx.x] += t;
     __syncthreads();
