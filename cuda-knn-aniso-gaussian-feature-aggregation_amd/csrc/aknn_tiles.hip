@@ -12,7 +12,7 @@
 //   3. One wave = 64 spatially sorted queries.  It visits the cloud's tiles outward
 //      from the tile nearest its first query; a tile is scanned only if some lane has
 //      mu_tile * dist^2(q, box) <= its current K-th key, and within it only the
-//      16-record quarters whose own box and mu pass the same test.  Skipped points
+//      8-record sub-tiles whose own box and mu pass the same test.  Skipped points
 //      provably have keys above every lane's K-th, so the kept set and order are those
 //      of a full scan.
 //
@@ -29,6 +29,8 @@
 namespace aknn {
 
 constexpr int kTile = 64;
+constexpr int kSubRec = 8;                 // records per sub-tile
+constexpr int kSubs = kTile / kSubRec;     // sub-tiles per tile
 constexpr int kSuper = 16;  // tiles per super-tile
 constexpr int kTileSentinel = kNoPoint;
 constexpr int kWaveSerialInserts = 16;  // per-query walk: more candidates in a tile -> sort
@@ -41,7 +43,7 @@ struct PrunedState {
   uint32_t *pperm, *qperm;
   int64_t *ptp, *qtp, *stp;
   void* rec;
-  float *hdr, *shdr, *sub;  // tile, super-tile and sub-tile (16 records) headers
+  float *hdr, *shdr, *sub;  // tile, super-tile and sub-tile (kSubRec records) headers
   int32_t* tcnt;
   void* tmp;
   size_t tmp_bytes;
@@ -239,7 +241,8 @@ __device__ double point_mu(const T* r9, const T* s3) {
 // ------------------------------------------------------------------ tile build
 // One wave per tile.  rec: [tiles*kTile][16] records; hdr: [tiles][8] =
 // {lo x y z, hi x y z, mu (float, rounded down; < 0 = always scan), first code (bits)};
-// sub: [tiles][4][8] = the same box and mu (last value unused) of each 16-record quarter.
+// sub: [tiles][kSubs][8] = the same box and mu (last value unused) of each kSubRec-record
+// sub-tile.
 template <typename T, int KEY>
 __global__ __launch_bounds__(64) void tile_build_kernel(
     const T* __restrict__ p, const T* __restrict__ sigma, const T* __restrict__ R, Clouds cl,
@@ -291,12 +294,12 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     if (!(mu >= 0.0)) bad = true;
   } else {
     // padding record: NaN position, so every key evaluated against it is NaN and it
-    // never enters a top-K list (the scan reads whole 16-record quarters)
+    // never enters a top-K list (the scan reads whole sub-tiles)
     for (int c = 0; c < 16; ++c) out[c] = c < 3 ? static_cast<T>(__builtin_nan("")) : T(0);
     if constexpr (sizeof(T) == 4) out[15] = __int_as_float(kTileSentinel);
     else out[15] = __longlong_as_double(static_cast<long long>(kTileSentinel));
   }
-  // Reductions over groups of 16 lanes (the sub-tiles), then over the wave (the tile).
+  // Reductions over groups of kSubRec lanes (the sub-tiles), then over the wave (the tile).
   // min / max are exact, so the grouping does not change any value.
   auto write_box = [&](float* h) {
     for (int c = 0; c < 3; ++c) {
@@ -318,8 +321,8 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     }
     mu = fmin(mu, __shfl_xor(mu, off, 64));
     bad = bad | (__shfl_xor(static_cast<int>(bad), off, 64) != 0);
-    if (off == 8 && (lane & 15) == 0) {
-      float* h = sub + (t * 4 + lane / 16) * 8;
+    if (off == kSubRec / 2 && (lane % kSubRec) == 0) {
+      float* h = sub + (t * kSubs + lane / kSubRec) * 8;
       write_box(h);
       h[7] = 0.f;
     }
@@ -542,33 +545,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         continue;
       ++n_scan;
       const int cnt = tcnt[t];
-      // the tile's four sub-tiles of 16 records: scan those some lane still needs
-      const float* __restrict__ sh = sub + static_cast<int64_t>(t) * 32;
+      // the tile's sub-tiles of kSubRec records: scan those some lane still needs
+      const float* __restrict__ sh = sub + static_cast<int64_t>(t) * (kSubs * 8);
       const T tt = fmin(key[KMAX - 1], thr);
       unsigned m = 0;
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if ((16 * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, tt, valid))) m |= 1u << u;
+      for (int u = 0; u < kSubs; ++u)
+        if ((kSubRec * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, tt, valid))) m |= 1u << u;
       if (m == 0) continue;
-      const int total = 16 * __builtin_popcount(m);
+      const int total = kSubRec * __builtin_popcount(m);
       n_pts += total;
       const T* __restrict__ base = rec + static_cast<int64_t>(t) * (kTile * 16);
-      // The needed quarters' records a pair at a time, as one stream; the next pair's
+      // The needed sub-tiles' records a pair at a time, as one stream; the next pair's
       // scalar loads are issued before the current pair is evaluated.  Scalar loads
       // complete out of order, so each wait is for everything issued: with this order
-      // it falls after the evaluation.  Quarters are whole (padding records are NaN).
+      // it falls after the evaluation.  Sub-tiles are whole (padding records are NaN).
       int u = __builtin_ctz(m), jj = 0;
-      Rec16<T> a0 = load_rec16(base + u * 256);
-      Rec16<T> a1 = load_rec16(base + u * 256 + 16);
+      Rec16<T> a0 = load_rec16(base + u * (kSubRec * 16));
+      Rec16<T> a1 = load_rec16(base + u * (kSubRec * 16) + 16);
 #pragma unroll 2
       for (int j = 0; j < total; j += 2) {
         int un = u, jn = jj + 2;
-        if (jn == 16) {
+        if (jn == kSubRec) {
           const unsigned rest = m & ~((2u << u) - 1u);
           un = rest ? __builtin_ctz(rest) : u;
-          jn = rest ? 0 : 14;  // past the end: reload the last pair (not evaluated)
+          jn = rest ? 0 : kSubRec - 2;  // past the end: reload the last pair (not evaluated)
         }
-        const T* nx = base + (un * 16 + jn) * 16;
+        const T* nx = base + (un * kSubRec + jn) * 16;
         const Rec16<T> b0 = load_rec16(nx);
         const Rec16<T> b1 = load_rec16(nx + 16);
         const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
@@ -1197,7 +1200,7 @@ static void carve_index(const PrunedLaunch& L, Carver& c, PrunedState& S) {
   S.rec = c.take(L.ntiles * kTile * 16 * el);
   S.hdr = reinterpret_cast<float*>(c.take(L.ntiles * 8 * 4));
   S.shdr = reinterpret_cast<float*>(c.take(L.nsuper * 8 * 4));
-  S.sub = reinterpret_cast<float*>(c.take(L.ntiles * 4 * 8 * 4));
+  S.sub = reinterpret_cast<float*>(c.take(L.ntiles * kSubs * 8 * 4));
   S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
 }
 
