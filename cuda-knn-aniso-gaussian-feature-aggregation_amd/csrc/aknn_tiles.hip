@@ -372,6 +372,60 @@ __device__ __forceinline__ void bubble_insert(T (&k)[N], int (&v)[N], T c, int c
   }
 }
 
+// A lane's top-K list of the tiled search, ascending in (key, index) order.
+// Generic form: separate key and index arrays, inserted by bubble_insert.
+template <typename T, int N>
+struct TopList {
+  T k[N];
+  int v[N];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      k[s] = __builtin_inf();
+      v[s] = kNoPoint;
+    }
+  }
+  __device__ __forceinline__ T last() const { return k[N - 1]; }
+  __device__ __forceinline__ void insert(T c, int ci) { bubble_insert<N>(k, v, c, ci); }
+  __device__ __forceinline__ T key(int s) const { return k[s]; }
+  __device__ __forceinline__ int id(int s) const { return v[s]; }
+};
+
+// FP32 form: one 64-bit word per entry, the order-preserving encoding of the key above
+// the (non-negative) index, so that unsigned 64-bit order is (key, index) order and an
+// insertion step is one 64-bit compare and two selects.  -0 is encoded as +0 (they
+// compare equal as keys); a -0 key is restored on output by the caller.  NaN keys are
+// never inserted.  `lastk` caches the decoded last key for the pruning tests.
+template <int N>
+struct TopList<float, N> {
+  uint64_t w[N];
+  float lastk;
+  static __device__ __forceinline__ uint64_t pack(float c, int ci) {
+    return (static_cast<uint64_t>(OrdKey<float>::enc(c + 0.0f)) << 32) | static_cast<uint32_t>(ci);
+  }
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int s = 0; s < N; ++s) w[s] = pack(__builtin_inff(), kNoPoint);
+    lastk = __builtin_inff();
+  }
+  __device__ __forceinline__ float last() const { return lastk; }
+  __device__ __forceinline__ void insert(float c, int ci) {
+    uint64_t x = pack(c, ci);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const uint64_t e = w[i];
+      const bool sw = x < e;
+      w[i] = sw ? x : e;
+      x = sw ? e : x;
+    }
+    lastk = key(N - 1);
+  }
+  __device__ __forceinline__ float key(int s) const {
+    return OrdKey<float>::dec(static_cast<uint32_t>(w[s] >> 32));
+  }
+  __device__ __forceinline__ int id(int s) const { return static_cast<int>(static_cast<uint32_t>(w[s])); }
+};
+
 // One 16-value point record, read with wave-uniform addresses (scalar loads).
 template <typename T>
 struct Rec16 {
@@ -447,7 +501,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     const T* __restrict__ rec, const float* __restrict__ hdr, const float* __restrict__ shdr,
     const float* __restrict__ sub, const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists,
     int64_t* __restrict__ idxs, unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
-    int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd) {
+    int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd, const T* __restrict__ pts,
+    const T* __restrict__ sig, const T* __restrict__ rot) {
   using Ord = OrdKey<T>;
   const int lane = threadIdx.x;
   int64_t b, q0;
@@ -484,13 +539,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     qz = q[row * 3 + 2];
   }
 
-  T key[KMAX];
-  int id[KMAX];
-#pragma unroll
-  for (int s = 0; s < KMAX; ++s) {
-    key[s] = __builtin_inf();
-    id[s] = kTileSentinel;
-  }
+  TopList<T, KMAX> L;
+  L.init();
 
   // this cloud's tiles [tb, te) and super-tiles [sb, se)
   // tile and super-tile numbers fit in 32 bits (the host checks the tile count); the
@@ -532,7 +582,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     ++n_super;
     if (ways > 1 && valid)
       thr = fmin(thr, Ord::dec(__hip_atomic_load(bnd + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-    if (!__any(box_needed<T>(shdr + static_cast<int64_t>(s) * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid)))
+    if (!__any(box_needed<T>(shdr + static_cast<int64_t>(s) * 8, qx, qy, qz, fmin(L.last(), thr), valid)))
       continue;
     const int t0 = tb + (s - sb) * kSuper;
     const int t1 = min(te, t0 + kSuper);
@@ -541,13 +591,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       const int t = to.next(tstep);
       if ((t - tb) % ways != iway) continue;
       ++n_tile;
-      if (!__any(box_needed<T>(hdr + static_cast<int64_t>(t) * 8, qx, qy, qz, fmin(key[KMAX - 1], thr), valid)))
+      if (!__any(box_needed<T>(hdr + static_cast<int64_t>(t) * 8, qx, qy, qz, fmin(L.last(), thr), valid)))
         continue;
       ++n_scan;
       const int cnt = tcnt[t];
       // the tile's sub-tiles of kSubRec records: scan those some lane still needs
       const float* __restrict__ sh = sub + static_cast<int64_t>(t) * (kSubs * 8);
-      const T tt = fmin(key[KMAX - 1], thr);
+      const T tt = fmin(L.last(), thr);
       unsigned m = 0;
 #pragma unroll
       for (int u = 0; u < kSubs; ++u)
@@ -577,30 +627,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
         const T c1 = rec_key<T, KEY>(qx, qy, qz, a1);
         // a key above the shared bound is above the final K-th key: never inserted
-        if (c0 <= fmin(key[KMAX - 1], thr)) bubble_insert<KMAX>(key, id, c0, rec_index(a0.v[15]));
-        if (c1 <= fmin(key[KMAX - 1], thr)) bubble_insert<KMAX>(key, id, c1, rec_index(a1.v[15]));
+        if (c0 <= fmin(L.last(), thr)) L.insert(c0, rec_index(a0.v[15]));
+        if (c1 <= fmin(L.last(), thr)) L.insert(c1, rec_index(a1.v[15]));
         a0 = b0;
         a1 = b1;
         u = un;
         jj = jn;
       }
-      if (ways > 1 && valid && key[KMAX - 1] < thr) {  // publish an improved bound
-        const T mine = key[KMAX - 1];
+      if (ways > 1 && valid && L.last() < thr) {  // publish an improved bound
+        const T mine = L.last();
         const typename Ord::U old = atomicMin(bnd + pos, Ord::enc(mine));
         thr = fmin(mine, Ord::dec(old));
       }
     }
   }
 
+  // The FP32 list holds a -0 key (possible only with sign-bit scales) as +0: a kept key
+  // of zero is recomputed from the point arrays, which gives the scan's exact bits.
+  const int64_t pb = cl.p_begin(b);
+  auto out_key = [&](int s) -> T {
+    T k = L.key(s);
+    if constexpr (sizeof(T) == 4 && KEY == AKNN_KEY_ANISO) {
+      const int n = L.id(s);
+      if (k == T(0) && n != kNoPoint) {
+        const int64_t r = pb + n;
+        k = pair_key<T, KEY>(qx, qy, qz, pts[r * 3], pts[r * 3 + 1], pts[r * 3 + 2], rot[r * 9],
+                             rot[r * 9 + 1], rot[r * 9 + 2], rot[r * 9 + 3], rot[r * 9 + 4],
+                             rot[r * 9 + 5], rot[r * 9 + 6], rot[r * 9 + 7], rot[r * 9 + 8], sig[r * 3],
+                             sig[r * 3 + 1], sig[r * 3 + 2]);
+      }
+    }
+    return k;
+  };
   if (valid) {
     if (ways == 1) {
       int last = 0;
 #pragma unroll
       for (int s = 0; s < KMAX; ++s) {
         if (s < K) {
-          dists[row * K + s] = key[s];
-          idxs[row * K + s] = static_cast<int64_t>(id[s]);
-          last = id[s];
+          dists[row * K + s] = out_key(s);
+          idxs[row * K + s] = static_cast<int64_t>(L.id(s));
+          last = L.id(s);
         }
       }
       if (last == kNoPoint) fill_unfilled_row<T>(dists + row * K, idxs + row * K, K);
@@ -610,8 +677,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
 #pragma unroll
       for (int s = 0; s < KMAX; ++s) {
         if (s < K) {
-          lk[s] = key[s];
-          li[s] = id[s];
+          lk[s] = out_key(s);
+          li[s] = L.id(s);
         }
       }
     }
@@ -1148,7 +1215,8 @@ static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, h
                      L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp, S.stp,
                      static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
                      static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
-                     static_cast<U*>(S.bnd));
+                     static_cast<U*>(S.bnd), static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
+                     static_cast<const T*>(L.R));
   if (ways > 1) {
     const int per = 64 / ways;
     hipLaunchKernelGGL((pruned_merge_kernel<T, KMAX>), dim3(static_cast<unsigned>((L.qrows + per - 1) / per)),

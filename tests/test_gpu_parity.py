@@ -133,6 +133,29 @@ def test_knn_tiles_degenerate_scales(cuda, search):
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("search,split", [("brute", 0), ("tiles", 0), ("tiles", 4), ("wave", 0)])
+def test_knn_signed_zero_keys(cuda, search, split, dtype):
+    """Keys of -0 (three points whose scales are all -0: every query's key to them is -0)
+    tie with keys of +0 (queries placed on positive-scale points, some with lower
+    indices): (key, index) order, and the sign bits of the keys exactly as scanned."""
+    b = workloads.clustered_batch(Q=600, P=5000, K=16, F=4, seed=61)
+    b = workloads.Batch(*(t.to(dtype) for t in (b.q, b.p, b.f, b.sigma, b.R)), b.K)
+    b.sigma[0, 10:13] = -0.0
+    b.q[0, 0:40] = b.p[0, 100:140]
+    b.q[0, 40:60] = b.p[0, 0:20]
+    tune = {"search": search}
+    if split:
+        tune["point_split"] = split
+    _, _, dists, idxs = fwd(b, cuda, tune=tune)
+    rd, ri = oracle_knn(b)
+    bits = torch.int32 if dtype == torch.float32 else torch.int64
+    neg0 = torch.tensor(-0.0, dtype=dtype).view(bits)
+    assert bool((rd.view(bits) == neg0).sum(-1).ge(3).all())   # the case is exercised
+    assert torch.equal(idxs.cpu(), ri)
+    assert torch.equal(dists.cpu().view(bits), rd.view(bits))
+
+
 @pytest.mark.parametrize("key", ["aniso", "euclidean"])
 @pytest.mark.parametrize("ways", [1, 2, 4, 16, 64])
 def test_knn_tiles_split_walk(cuda, ways, key):
