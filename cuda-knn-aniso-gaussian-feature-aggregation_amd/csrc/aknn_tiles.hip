@@ -3,7 +3,7 @@
 // Same result as the brute-force scan (csrc/forward.cu:83-97 semantics, (key, index)
 // order) in far fewer key evaluations.  Per call:
 //
-//   1. cloud boxes -> 30-bit Morton code per point and per query (10 bits per axis in
+//   1. cloud boxes -> 24-bit Morton code per point and per query (8 bits per axis in
 //      the cloud's point box); stable radix sorts give a spatial order of both.
 //   2. Points are regrouped in that order into tiles of kTile records
 //      {p0 p1 p2, R00..R22, s0 s1 s2, original index}; each tile keeps its box and
@@ -491,10 +491,19 @@ struct Outward {
   }
 };
 
+// Occupancy target of the tiled search: its list (KMAX keys and indices, with the copy
+// the insertion chain keeps) must fit in the VGPR budget, or it spills to scratch.
+template <typename T, int KMAX>
+constexpr int search_waves() {
+  if constexpr (sizeof(T) == 4) return KMAX <= 16 ? 6 : KMAX <= 32 ? 3 : 2;
+  else return KMAX <= 8 ? 6 : KMAX <= 16 ? 4 : KMAX <= 32 ? 2 : 1;
+}
+
 // grid.x = query tiles of 64 sorted queries (ragged: via the query tile prefix),
 // grid.y = clouds (dense).  Two-level walk: super-tiles of kSuper tiles, then tiles.
 template <typename T, int KMAX, int KEY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void knn_pruned_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves<T, KMAX>(),
+                                                                   search_waves<T, KMAX>()))) void knn_pruned_kernel(
     const T* __restrict__ q, Clouds cl, const int64_t* __restrict__ qtiles,
     const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys,
     const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
