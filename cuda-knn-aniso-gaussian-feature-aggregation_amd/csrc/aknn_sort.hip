@@ -196,15 +196,27 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   gpos[threadIdx.x] = start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
   __syncthreads();
 
+  // all of this thread's keys and values are loaded up front, so their loads are in
+  // flight together rather than one chunk at a time behind the barriers
+  K kreg[kSortItems];
+  uint32_t vreg[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const int jj = i * kSortThreads + threadIdx.x;
+    const bool ok = jj < tile_n;
+    kreg[i] = ok ? kin[base + jj] : K(0);
+    vreg[i] = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
+  }
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
 #pragma unroll
     for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
     const int jj = i * kSortThreads + threadIdx.x;
     const bool ok = jj < tile_n;
-    const K key = ok ? kin[base + jj] : K(0);
-    const uint32_t val = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
+    const K key = kreg[i];
+    const uint32_t val = vreg[i];
     const uint32_t d = ok ? static_cast<uint32_t>((key >> shift) & 255u) : 256u;
     unsigned long long same = __ballot(ok);
 #pragma unroll
