@@ -393,7 +393,7 @@ struct TopList {
 
 // FP32 form: one 64-bit word per entry, the order-preserving encoding of the key above
 // the (non-negative) index, so that unsigned 64-bit order is (key, index) order and an
-// insertion step is one 64-bit compare and two selects.  -0 is encoded as +0 (they
+// insertion step is one 64-bit compare and two 64-bit selects, in place.  -0 is encoded as +0 (they
 // compare equal as keys); a -0 key is restored on output by the caller.  NaN keys are
 // never inserted.  `lastk` caches the decoded last key for the pruning tests.
 template <int N>
@@ -410,13 +410,15 @@ struct TopList<float, N> {
   }
   __device__ __forceinline__ float last() const { return lastk; }
   __device__ __forceinline__ void insert(float c, int ci) {
-    uint64_t x = pack(c, ci);
+    const uint64_t x = pack(c, ci);
+    // shift insertion, last slot first: slot i takes slot i-1 when x belongs before
+    // it, x when x lands at i, else keeps its word (the words are distinct and sorted,
+    // so the compares are monotone in i)
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const uint64_t e = w[i];
-      const bool sw = x < e;
-      w[i] = sw ? x : e;
-      x = sw ? e : x;
+    for (int i = N - 1; i >= 0; --i) {
+      const bool here = x < w[i];
+      const bool before = i > 0 && x < w[i > 0 ? i - 1 : 0];
+      w[i] = before ? w[i > 0 ? i - 1 : 0] : (here ? x : w[i]);
     }
     lastk = key(N - 1);
   }
