@@ -608,11 +608,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
       const int cnt = tcnt[t];
       // the tile's sub-tiles of kSubRec records: scan those some lane still needs
       const float* __restrict__ sh = sub + static_cast<int64_t>(t) * (kSubs * 8);
-      const T tt = fmin(L.last(), thr);
+      // lim = fmin(last kept key, thr): thr is fixed within a tile, so lim changes only
+      // with an insertion
+      T lim = fmin(L.last(), thr);
       unsigned m = 0;
 #pragma unroll
       for (int u = 0; u < kSubs; ++u)
-        if ((kSubRec * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, tt, valid))) m |= 1u << u;
+        if ((kSubRec * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, lim, valid))) m |= 1u << u;
       if (m == 0) continue;
       const int total = kSubRec * __builtin_popcount(m);
       n_pts += total;
@@ -621,29 +623,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
       // scalar loads are issued before the current pair is evaluated.  Scalar loads
       // complete out of order, so each wait is for everything issued: with this order
       // it falls after the evaluation.  Sub-tiles are whole (padding records are NaN).
-      int u = __builtin_ctz(m), jj = 0;
-      Rec16<T> a0 = load_rec16(base + u * (kSubRec * 16));
-      Rec16<T> a1 = load_rec16(base + u * (kSubRec * 16) + 16);
-#pragma unroll 2
-      for (int j = 0; j < total; j += 2) {
-        int un = u, jn = jj + 2;
-        if (jn == kSubRec) {
-          const unsigned rest = m & ~((2u << u) - 1u);
-          un = rest ? __builtin_ctz(rest) : u;
-          jn = rest ? 0 : kSubRec - 2;  // past the end: reload the last pair (not evaluated)
+      // The pairs of a sub-tile are unrolled, so their offsets are constants and the
+      // stream's bookkeeping runs once per sub-tile.
+      unsigned rest = m;
+      const T* cur = base + __builtin_ctz(rest) * (kSubRec * 16);
+      rest &= rest - 1;
+      Rec16<T> a0 = load_rec16(cur);
+      Rec16<T> a1 = load_rec16(cur + 16);
+      while (true) {
+        // the next sub-tile (past the end: the current one again, loaded, not evaluated)
+        const T* nxt = rest ? base + __builtin_ctz(rest) * (kSubRec * 16) : cur;
+#pragma unroll
+        for (int j = 0; j < kSubRec; j += 2) {
+          const T* nx = j + 2 < kSubRec ? cur + (j + 2) * 16 : nxt;
+          const Rec16<T> b0 = load_rec16(nx);
+          const Rec16<T> b1 = load_rec16(nx + 16);
+          const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
+          const T c1 = rec_key<T, KEY>(qx, qy, qz, a1);
+          // a key above the shared bound is above the final K-th key: never inserted
+          if (c0 <= lim) {
+            L.insert(c0, rec_index(a0.v[15]));
+            lim = fmin(L.last(), thr);
+          }
+          if (c1 <= lim) {
+            L.insert(c1, rec_index(a1.v[15]));
+            lim = fmin(L.last(), thr);
+          }
+          a0 = b0;
+          a1 = b1;
         }
-        const T* nx = base + (un * kSubRec + jn) * 16;
-        const Rec16<T> b0 = load_rec16(nx);
-        const Rec16<T> b1 = load_rec16(nx + 16);
-        const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
-        const T c1 = rec_key<T, KEY>(qx, qy, qz, a1);
-        // a key above the shared bound is above the final K-th key: never inserted
-        if (c0 <= fmin(L.last(), thr)) L.insert(c0, rec_index(a0.v[15]));
-        if (c1 <= fmin(L.last(), thr)) L.insert(c1, rec_index(a1.v[15]));
-        a0 = b0;
-        a1 = b1;
-        u = un;
-        jj = jn;
+        if (!rest) break;
+        rest &= rest - 1;
+        cur = nxt;
       }
       if (ways > 1 && valid && L.last() < thr) {  // publish an improved bound
         const T mine = L.last();
