@@ -111,6 +111,12 @@ int check_dims(const aknn_dims* d, bool need_k) {
   if (d->B < 0 || d->Q < 0 || d->P < 0 || d->F < 1)
     return fail(AKNN_E_INVALID, "bad sizes B=%lld Q=%lld P=%lld F=%lld", (long long)d->B,
                 (long long)d->Q, (long long)d->P, (long long)d->F);
+  // The gather kernels hold a feature row in registers: 16 lanes x up to 8 chunks of the
+  // widest vector (at most 16 B) that divides F.
+  const int64_t fvec = (d->dtype == AKNN_F32 && d->F % 4 == 0) ? 4 : (d->F % 2 == 0 ? 2 : 1);
+  if (d->F > 128 * fvec)
+    return fail(AKNN_E_UNSUPPORTED, "F must be <= %lld for this F (128 x the widest vector of <= 16 B "
+                "dividing F; got %lld)", (long long)(128 * fvec), (long long)d->F);
   const bool rq = d->q_off != nullptr, rp = d->p_off != nullptr;
   if (rq != rp || (rq && (!d->q_off_host || !d->p_off_host)))
     return fail(AKNN_E_INVALID, "ragged batches need device and host offsets for q and p");

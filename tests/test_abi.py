@@ -42,6 +42,9 @@ def _dims(**kw):
     (dict(K=9), _lib.AKNN_E_INVALID, "must not exceed"),
     (dict(K=65, P=100), _lib.AKNN_E_UNSUPPORTED, "K must be <= 64"),
     (dict(F=0), _lib.AKNN_E_INVALID, "bad sizes"),
+    (dict(F=516), _lib.AKNN_E_UNSUPPORTED, "F must be <= 512"),
+    (dict(F=129), _lib.AKNN_E_UNSUPPORTED, "F must be <= 128"),
+    (dict(F=258, dtype=torch.float64), _lib.AKNN_E_UNSUPPORTED, "F must be <= 256"),
 ])
 def test_forward_rejects_bad_dims(kw, code, msg):
     lib = _lib.load()

@@ -378,6 +378,35 @@ def test_autograd_eikonal_step_matches_oracle(cuda):
         assert_rel("grad " + n, x.grad, y, TOL32)
 
 
+@pytest.mark.parametrize("F", [96, 127, 130, 256, 512])
+def test_wide_feature_rows(cuda, F):
+    """Feature rows held as 2 to 8 register chunks per 16-lane group (F = 96: two float4
+    chunks; 127: eight scalar; 130: five float2 in the 8-chunk kernel; 512: the widest
+    supported): aggregation, first order and the Eikonal step vs the FP64 oracle."""
+    b = workloads.uniform_batch(1, 400, 300, 16, F, seed=70 + F, sigma_range=(2.0, 40.0))
+    ins = [t.to(cuda).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    f_out, w_out, dists, idxs = knn_aggregate_aniso(*ins, b.K)
+    ref_out = plain.forward_given_idxs(*f64(b), idxs.cpu())[0]
+    assert_rel("f_out", f_out, ref_out, TOL32)
+    (f_out * b.g.to(cuda)).sum().backward()
+    ref = plain.backward_closed(b.g.double(), *f64(b), idxs.cpu())
+    for n, x, y in zip(("q", "p", "f", "sigma", "R"), ins, ref):
+        assert_rel("grad " + n, x.grad, y, TOL32)
+        x.grad = None
+    f_out = knn_aggregate_aniso(*ins, b.K)[0]
+    dq = torch.autograd.grad(f_out, ins[0], b.v.to(cuda).expand_as(f_out), create_graph=True)[0]
+    L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
+    L.backward()
+
+    def sel(q, p, s, R, K, key="aniso"):
+        return None, idxs.cpu()
+
+    Lr, gr = plain.eikonal_step(*f64(b), b.K, b.v.double(), select=sel)
+    assert abs(L.item() - Lr.item()) <= TOL32 * abs(Lr.item())
+    for n, x, y in zip(("q", "p", "f", "sigma", "R"), ins, gr):
+        assert_rel("eikonal grad " + n, x.grad, y, TOL32)
+
+
 def test_autograd_first_order_matches_oracle(cuda):
     b = workloads.readme(seed=3)
     ins = [t.to(cuda).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
