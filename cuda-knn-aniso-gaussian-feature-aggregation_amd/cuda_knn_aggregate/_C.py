@@ -152,6 +152,37 @@ def _cp(s):
 
 
 # ----------------------------------------------------------------------------- forward
+def _feature_slabs(F: int, dt: torch.dtype):
+    """Column ranges ``[s0, s1)`` splitting a feature row wider than the library takes in
+    one call, or ``None`` when ``F`` fits.  The library holds a row in registers: up to
+    128 times the widest vector (at most 16 B) that divides the width, so 512 (FP32,
+    width divisible by 4), 256 (even) or 128.  Every term of the forward and of both
+    backward orders is either local to a feature column or linear in the row dot
+    products (``sum_F g f`` and the like), so per-range calls add up to the whole row:
+    column outputs are placed side by side, geometric gradients summed in range order.
+    """
+    def cap(n: int) -> int:
+        v = 4 if (dt == torch.float32 and n % 4 == 0) else (2 if n % 2 == 0 else 1)
+        return 128 * v
+
+    if F <= cap(F):
+        return None
+    width = 512 if dt == torch.float32 else 256
+    out, s0 = [], 0
+    while F - s0 > width:
+        out.append((s0, s0 + width))
+        s0 += width
+    if F - s0 <= cap(F - s0):
+        out.append((s0, F))
+    else:
+        out += [(a, min(a + 128, F)) for a in range(s0, F, 128)]
+    return out
+
+
+def _cols(t, s0, s1):
+    return None if t is None else t[..., s0:s1].contiguous()
+
+
 def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tune=None, cache=None):
     """Neighbour search + aggregation -> ``[f_out, w_out, dists, idxs]``.
 
@@ -165,6 +196,17 @@ def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tun
     dt = _check_float((q, p, f, sigma, R), ("q", "p", "f", "sigma", "R"))
     B, Q, P, F = _check_shapes(q, p, f, sigma, R)
     K = _check_k(K, P)
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:  # one search, then the other column ranges on its neighbours
+        (s0, s1), rest = slabs[0], slabs[1:]
+        f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
+        first, w_out, dists, idxs = knn_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, K, key=key,
+                                                                tune=tune, cache=cache)
+        f_out[..., s0:s1] = first
+        for s0, s1 in rest:
+            f_out[..., s0:s1] = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, idxs, K,
+                                                                        key=key)[0]
+        return [f_out, w_out, dists, idxs]
     dev = q.device
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
     w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
@@ -202,6 +244,15 @@ def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, k
     if int(knn_idxs.shape[2]) != int(K):
         raise RuntimeError(f"knn_idxs has {knn_idxs.shape[2]} neighbours but K = {K}")
     K = _check_k(K, P)
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:
+        f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
+        for i, (s0, s1) in enumerate(slabs):
+            o = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_idxs, K, key=key)
+            f_out[..., s0:s1] = o[0]
+            if i == 0:
+                w_out, dists = o[1], o[2]
+        return [f_out, w_out, dists]
     dev = q.device
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
     w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
@@ -242,6 +293,15 @@ def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
     K = _check_k(K, P)
     if int(knn_table.shape[1]) < K:
         raise RuntimeError(f"knn_table has {knn_table.shape[1]} columns but K = {K}")
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:  # one lookup, then the other column ranges on its neighbours
+        (s0, s1), rest = slabs[0], slabs[1:]
+        f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
+        first, w_out, dists, idxs = knn_lookup_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_table, K)
+        f_out[..., s0:s1] = first
+        for s0, s1 in rest:
+            f_out[..., s0:s1] = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, idxs, K)[0]
+        return [f_out, w_out, dists, idxs]
     dev = q.device
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
     w_out = torch.empty(B, Q, K, dtype=dt, device=dev)
@@ -288,6 +348,15 @@ def knn_aggregate_forward(q, p, f, sigma, K, *, tune=None):
         raise RuntimeError(f"sigma must be [B, P, C] with C >= 1 (got {tuple(sigma.shape)})")
     F = f.shape[2]
     K = _check_k(K, P)
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:  # each column range searches again (same neighbours, same weights)
+        f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
+        for i, (s0, s1) in enumerate(slabs):
+            o = knn_aggregate_forward(q, p, _cols(f, s0, s1), sigma, K, tune=tune)
+            f_out[..., s0:s1] = o[0]
+            if i == 0:
+                w_out, dists, idxs = o[1], o[2], o[3]
+        return [f_out, w_out, dists, idxs]
     dev = q.device
     scale = sigma[..., 0].contiguous()
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
@@ -433,6 +502,25 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
         raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
     names = ("dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:
+        if mask & _lib.NEED_POINTS and csr is None:
+            csr = build_csr(k_idxs, P, key)
+        df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
+        res = None
+        for s0, s1 in slabs:
+            o = knn_aggregate_aniso_backward(_cols(grad_f_out, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R,
+                                             w_out, k_idxs, key=key, csr=csr, needs=needs, cache=cache)
+            if df is not None:
+                df[..., s0:s1] = o[2]
+            if res is None:
+                res = o
+            else:
+                for i in (0, 1, 3, 4):
+                    if res[i] is not None:
+                        res[i] += o[i]
+        res[2] = df
+        return res
     dev = q.device
     outs = {
         "dq": torch.empty(B, Q, 3, dtype=dt, device=dev) if mask & _lib.NEED_DQ else None,
@@ -490,6 +578,30 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
             raise RuntimeError(f"{n} must have shape {list(want)} (got {tuple(t.shape)})")
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    slabs = _feature_slabs(F, dt)
+    if slabs is not None:
+        if (mask & _lib.NEED_POINTS or (full and mask & _lib.NEED_DQ)) and csr is None:
+            csr = build_csr(k_idxs, P, key)
+        dgf = torch.empty(B, Q, F, dtype=dt, device=q.device) if mask & _lib.NEED_DGRAD_F_OUT else None
+        df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
+        res = None
+        for s0, s1 in slabs:
+            o = knn_aggregate_aniso_backward_2nd(
+                grad_grad_q, _cols(grad_f_out, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R, w_out, k_idxs,
+                key=key, csr=csr, needs=needs, cache=cache, grad_grad_p=grad_grad_p,
+                grad_grad_f=_cols(grad_grad_f, s0, s1), grad_grad_sigma=grad_grad_sigma, grad_grad_R=grad_grad_R)
+            if dgf is not None:
+                dgf[..., s0:s1] = o[0]
+            if df is not None:
+                df[..., s0:s1] = o[3]
+            if res is None:
+                res = o
+            else:
+                for i in (1, 2, 4, 5):
+                    if res[i] is not None:
+                        res[i] += o[i]
+        res[0], res[3] = dgf, df
+        return res
     dev = q.device
     outs = {
         "dgf": torch.empty(B, Q, F, dtype=dt, device=dev) if mask & _lib.NEED_DGRAD_F_OUT else None,

@@ -83,6 +83,15 @@ def forward(layout, q, p, f, sigma, R, K, *, key="aniso", tune=None, cache=None)
     """-> ``[f_out [Q,F], w_out [Q,K], dists [Q,K], idxs [Q,K] (int64, within-cloud)]``."""
     dt, F = _check(layout, q, p, f, sigma, R)
     K = _check_k(layout, K)
+    slabs = _C._feature_slabs(F, dt)
+    if slabs is not None:  # feature rows wider than one call: column ranges (each searches again)
+        f_out = torch.empty(layout.Q, F, dtype=dt, device=q.device)
+        for i, (s0, s1) in enumerate(slabs):
+            o = forward(layout, q, p, _C._cols(f, s0, s1), sigma, R, K, key=key, tune=tune, cache=cache)
+            f_out[:, s0:s1] = o[0]
+            if i == 0:
+                w_out, dists, idxs = o[1], o[2], o[3]
+        return [f_out, w_out, dists, idxs]
     dev = q.device
     f_out = torch.empty(layout.Q, F, dtype=dt, device=dev)
     w_out = torch.empty(layout.Q, K, dtype=dt, device=dev)
@@ -125,6 +134,28 @@ def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, nee
     K = int(idxs.shape[1])
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR") if second else ("dq", "dp", "df", "dsigma", "dR")
     mask = _C._needs_mask(needs, names)
+    slabs = _C._feature_slabs(F, dt)
+    if slabs is not None:  # column ranges: column outputs side by side, the rest summed in order
+        if mask & _lib.NEED_POINTS and csr is None:
+            csr = build_csr(layout, idxs, K)
+        cols = {n: (torch.empty(layout.Q if n == "dgf" else layout.P, F, dtype=dt, device=q.device)
+                    if mask & _C._NEED_BITS[n] else None) for n in ("dgf", "df") if n in names}
+        res = None
+        for s0, s1 in slabs:
+            o = _backward_impl(layout, second, gg, _C._cols(g, s0, s1), q, p, _C._cols(f, s0, s1), sigma, R, idxs,
+                               key, csr, needs, cache)
+            for n, t in cols.items():
+                if t is not None:
+                    t[:, s0:s1] = o[names.index(n)]
+            if res is None:
+                res = o
+            else:
+                for i, n in enumerate(names):
+                    if n not in cols and res[i] is not None:
+                        res[i] += o[i]
+        for n, t in cols.items():
+            res[names.index(n)] = t
+        return res
     dev = q.device
     Q, P = layout.Q, layout.P
     shapes = {"dgf": (Q, F), "dq": (Q, 3), "dp": (P, 3), "df": (P, F), "dsigma": (P, 3), "dR": (P, 3, 3)}

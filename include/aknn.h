@@ -53,7 +53,10 @@ extern "C" {
 #define AKNN_E_INVALID -1   /* bad argument (shape, K range, null pointer, ...) */
 #define AKNN_E_WORKSPACE -2 /* workspace too small */
 #define AKNN_E_LAUNCH -3    /* HIP launch / runtime error */
-#define AKNN_E_UNSUPPORTED -4
+#define AKNN_E_UNSUPPORTED -4 /* beyond this build's limits: K > 64, batch rows >= 2^31, or a feature row
+                                wider than one call takes (F > 128 x the widest vector of <= 16 B dividing F:
+                                512 in FP32 for F % 4 == 0, 256 for even F, else 128; split wider rows into
+                                column ranges, as the Python layer does) */
 
 /* dtype */
 #define AKNN_F32 0

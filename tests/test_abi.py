@@ -100,3 +100,24 @@ def test_python_layer_rejects_cpu_and_noncontiguous():
     q = torch.rand(1, 4, 3)
     with pytest.raises(RuntimeError, match="CUDA tensor"):
         _C.knn_aggregate_aniso_forward(q, q, q, q, torch.rand(1, 4, 3, 3), 2)
+
+
+@pytest.mark.parametrize("F,dt", [(512, torch.float32), (513, torch.float32), (1030, torch.float32),
+                                  (1031, torch.float32), (129, torch.float32), (257, torch.float64),
+                                  (600, torch.float64), (64, torch.float64)])
+def test_feature_slabs_cover_the_row(F, dt):
+    """Wide feature rows are split into contiguous column ranges, each within the
+    library's per-call width (the same rule its argument check applies)."""
+    slabs = _C._feature_slabs(F, dt)
+
+    def cap(n):
+        return 128 * (4 if dt == torch.float32 and n % 4 == 0 else 2 if n % 2 == 0 else 1)
+
+    if F <= cap(F):
+        assert slabs is None
+        return
+    assert slabs[0][0] == 0 and slabs[-1][1] == F
+    for (a0, a1), (b0, _) in zip(slabs, slabs[1:]):
+        assert a1 == b0
+    for a0, a1 in slabs:
+        assert 0 < a1 - a0 <= cap(a1 - a0)

@@ -378,11 +378,12 @@ def test_autograd_eikonal_step_matches_oracle(cuda):
         assert_rel("grad " + n, x.grad, y, TOL32)
 
 
-@pytest.mark.parametrize("F", [96, 127, 130, 256, 512])
+@pytest.mark.parametrize("F", [96, 127, 130, 256, 512, 1030])
 def test_wide_feature_rows(cuda, F):
     """Feature rows held as 2 to 8 register chunks per 16-lane group (F = 96: two float4
     chunks; 127: eight scalar; 130: five float2 in the 8-chunk kernel; 512: the widest
-    supported): aggregation, first order and the Eikonal step vs the FP64 oracle."""
+    one call takes; 1030: three column ranges of 512, 512 and 6): aggregation, first
+    order and the Eikonal step vs the FP64 oracle."""
     b = workloads.uniform_batch(1, 400, 300, 16, F, seed=70 + F, sigma_range=(2.0, 40.0))
     ins = [t.to(cuda).requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
     f_out, w_out, dists, idxs = knn_aggregate_aniso(*ins, b.K)
@@ -405,6 +406,24 @@ def test_wide_feature_rows(cuda, F):
     assert abs(L.item() - Lr.item()) <= TOL32 * abs(Lr.item())
     for n, x, y in zip(("q", "p", "f", "sigma", "R"), ins, gr):
         assert_rel("eikonal grad " + n, x.grad, y, TOL32)
+
+
+def test_wide_feature_rows_fp64(cuda):
+    """FP64 rows wider than one call (F = 300: column ranges of 256 and 44): forward and
+    both backward orders vs the closed forms, to the FP64 tolerance."""
+    b = workloads.uniform_batch(1, 300, 200, 8, 300, seed=91, sigma_range=(2.0, 40.0))
+    x = [t.to(cuda, torch.float64) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+    f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(*x, b.K)
+    ib = idxs.cpu()
+    assert_rel("f_out", f_out, plain.forward_given_idxs(*f64(b), ib)[0], TOL64)
+    g = b.g.double()
+    h = torch.randn(1, 300, 3, dtype=torch.float64, generator=torch.Generator().manual_seed(92))
+    got1 = _C.knn_aggregate_aniso_backward(g.to(cuda), None, *x, w_out, idxs)
+    for n, u, v in zip(("dq", "dp", "df", "dsigma", "dR"), got1, plain.backward_closed(g, *f64(b), ib)):
+        assert_rel(n, u, v, TOL64)
+    got2 = _C.knn_aggregate_aniso_backward_2nd(h.to(cuda), g.to(cuda), None, *x, w_out, idxs)
+    for n, u, v in zip(("dgf", "dq", "dp", "df", "dsigma", "dR"), got2, plain.backward2_closed(h, g, *f64(b), ib)):
+        assert_rel(n, u, v, TOL64)
 
 
 def test_autograd_first_order_matches_oracle(cuda):

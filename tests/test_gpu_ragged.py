@@ -78,6 +78,36 @@ def test_ragged_backward_equals_per_cloud(cuda):
             assert plain.rel_err(x[rows[0]:rows[1]].cpu()[None], y) < 1e-5, (b, n)
 
 
+def test_ragged_wide_feature_rows(cuda):
+    """Feature rows wider than one library call (F = 600: column ranges of 512 and 88)
+    through the ragged forward and both backward orders, vs the FP64 oracle per cloud."""
+    r = small_ragged(3)
+    F = 600
+    r.f = torch.rand(r.f.shape[0], F, generator=torch.Generator().manual_seed(33))
+    layout = RaggedLayout(r.q_off, r.p_off, cuda)
+    d = r.to(cuda)
+    f_out, w_out, dists, idxs = rg.forward(layout, d.q, d.p, d.f, d.sigma, d.R, r.K)
+    g = torch.randn(layout.Q, F, device=cuda)
+    h = torch.randn(layout.Q, 3, device=cuda)
+    g1 = rg.backward(layout, g, d.q, d.p, d.f, d.sigma, d.R, idxs)
+    g2 = rg.backward2(layout, h, g, d.q, d.p, d.f, d.sigma, d.R, idxs)
+    for b in range(r.B):
+        qs, qe = int(r.q_off[b]), int(r.q_off[b + 1])
+        ps, pe = int(r.p_off[b]), int(r.p_off[b + 1])
+        c = r.cloud(b)
+        ins = [t.double() for t in (c.q, c.p, c.f, c.sigma, c.R)]
+        ib = idxs[qs:qe].cpu()[None]
+        assert plain.rel_err(f_out[qs:qe].cpu()[None], plain.forward_given_idxs(*ins, ib)[0]) < 1e-5, b
+        ref1 = plain.backward_closed(g[qs:qe].cpu().double()[None], *ins, ib)
+        for n, x, y, rows in zip(("dq", "dp", "df", "dsigma", "dR"), g1, ref1,
+                                 [(qs, qe)] + [(ps, pe)] * 4):
+            assert plain.rel_err(x[rows[0]:rows[1]].cpu()[None], y) < 1e-5, (b, n)
+        ref2 = plain.backward2_closed(h[qs:qe].cpu().double()[None], g[qs:qe].cpu().double()[None], *ins, ib)
+        for n, x, y, rows in zip(("dgf", "dq", "dp", "df", "dsigma", "dR"), g2, ref2,
+                                 [(qs, qe), (qs, qe)] + [(ps, pe)] * 4):
+            assert plain.rel_err(x[rows[0]:rows[1]].cpu()[None], y) < 1e-5, (b, n)
+
+
 def eikonal_step_ragged(params, layout, K, v):
     f_out, *_ = knn_aggregate_aniso_ragged(*params, layout, K)
     dq = torch.autograd.grad(f_out, params[0], v.expand_as(f_out), create_graph=True)[0]
