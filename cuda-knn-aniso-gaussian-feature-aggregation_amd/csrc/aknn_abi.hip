@@ -53,6 +53,10 @@ class Span {
     if (rec_.a && rec_.b) {
       on_ = hipEventRecord(rec_.a, st_) == hipSuccess;  // tracing only: a failed record drops the span
     }
+    if (!on_) {  // the events go back to the pool
+      if (rec_.a) g_event_pool.push_back(rec_.a);
+      if (rec_.b) g_event_pool.push_back(rec_.b);
+    }
   }
   ~Span() {
     if (!on_) return;
