@@ -211,16 +211,16 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   if (ragged(d)) {
     for (int64_t b = 0; b < d->B; ++b) {
       const int64_t pb = d->p_off_host[b + 1] - d->p_off_host[b];
-      const int64_t tiles = ceil_div(pb, 64);
+      const int64_t tiles = ceil_div(pb, aknn::kTile);
       L.ntiles += tiles;
-      L.nsuper += ceil_div(tiles, 16);
-      L.nqtiles += ceil_div(d->q_off_host[b + 1] - d->q_off_host[b], 64);
+      L.nsuper += ceil_div(tiles, aknn::kSuper);
+      L.nqtiles += ceil_div(d->q_off_host[b + 1] - d->q_off_host[b], aknn::kQueryTile);
       L.max_points = std::max(L.max_points, pb);
     }
   } else {
-    L.ntiles = d->B * ceil_div(d->P, 64);
-    L.nsuper = d->B * ceil_div(ceil_div(d->P, 64), 16);
-    L.nqtiles = d->B * ceil_div(d->Q, 64);
+    L.ntiles = d->B * ceil_div(d->P, aknn::kTile);
+    L.nsuper = d->B * ceil_div(ceil_div(d->P, aknn::kTile), aknn::kSuper);
+    L.nqtiles = d->B * ceil_div(d->Q, aknn::kQueryTile);
     L.max_points = d->P;
   }
   // Few query tiles (small or scattered query sets) cannot fill the device and the

@@ -21,13 +21,19 @@ struct KnnLaunch {
   int64_t* idxs;
 };
 
+// Point tiles of the tiled search and the per-query walk: records per tile, and tiles
+// per super-tile.  Queries go in tiles of 64 (one per lane of a wave).
+constexpr int kTile = 64;
+constexpr int kSuper = 16;
+constexpr int kQueryTile = 64;
+
 // Tile-pruned top-K (same results as KnnLaunch's brute-force scan).
 struct PrunedLaunch {
   int32_t dtype, key;
   Clouds cl;
   int64_t qrows, prows;   // query and point rows of the batch
   int64_t ntiles;         // point tiles (sum over clouds of ceil(P_b / 64))
-  int64_t nsuper;         // super-tiles (sum over clouds of ceil(tiles_b / 16))
+  int64_t nsuper;         // super-tiles (sum over clouds of ceil(tiles_b / kSuper))
   int64_t nqtiles;        // query tiles (ragged: sum of ceil(Q_b / 64))
   int64_t max_points;     // largest cloud (sizes the box reduction grid)
   int32_t ways;           // blocks sharing one query tile's walk (power of two <= 64)

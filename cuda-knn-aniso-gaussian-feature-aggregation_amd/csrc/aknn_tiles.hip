@@ -28,10 +28,8 @@
 
 namespace aknn {
 
-constexpr int kTile = 64;
 constexpr int kSubRec = 8;                 // records per sub-tile
 constexpr int kSubs = kTile / kSubRec;     // sub-tiles per tile
-constexpr int kSuper = 16;  // tiles per super-tile
 constexpr int kTileSentinel = kNoPoint;
 constexpr int kWaveSerialInserts = 16;  // per-query walk: more candidates in a tile -> sort
 
