@@ -7,8 +7,10 @@
 //
 // Per pass over digit (key >> shift) & 255, with tiles of kTileKeys keys per block:
 //   1. digit_hist: per-tile digit counts (LDS), stored digit-major: counts[d * tiles + t]
-//   2. exclusive scan of counts (3 kernels): start of each (digit, tile) in the output
-//   3. digit_scatter: local digit starts from the tile's own counts (kept by step 1);
+//   2. digit_scan: one block per digit scans that digit's tile counts (in place) and
+//      records the digit's total
+//   3. digit_scatter: the digit bases (a scan of the 256 totals, per block), local digit
+//      starts from the tile's own counts (kept by step 1);
 //      each tile walks its keys in order, 256 at a time; a key's rank
 //      among equal digits is (earlier chunks) + (earlier waves of this chunk) +
 //      (earlier lanes of its wave, from 8 ballots over the digit bits).  Keys are
@@ -21,7 +23,6 @@ namespace aknn {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kTileKeys = kSortThreads * kSortItems;  // 4096
-constexpr int kScanChunk = 4096;                      // elements per scan block
 
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __restrict__ keys, int64_t n,
@@ -42,26 +43,8 @@ __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __res
   tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];  // kept per tile
 }
 
-// ---- exclusive scan of uint32 (in place), three kernels
-__global__ __launch_bounds__(256) void scan_reduce_kernel(const uint32_t* __restrict__ x, int64_t n,
-                                                          uint32_t* __restrict__ sums) {
-  __shared__ uint32_t s[256];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk;
-  uint32_t acc = 0;
-  for (int i = threadIdx.x; i < kScanChunk; i += 256) {
-    const int64_t j = base + i;
-    if (j < n) acc += x[j];
-  }
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
-}
-
-// Block-wide exclusive scan of 256 values (Hillis-Steele in LDS); returns the total.
+// ---- per-digit scan of the tile counts
+// Exclusive scan of 256 values over the block (256 threads); returns the total.
 __device__ __forceinline__ uint32_t block_exscan256(uint32_t* s, uint32_t v, uint32_t* out) {
   s[threadIdx.x] = v;
   __syncthreads();
@@ -78,93 +61,35 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t* s, uint32_t v, uin
   return total;
 }
 
-__global__ __launch_bounds__(256) void scan_top_kernel(uint32_t* __restrict__ sums, int64_t m) {
+// One block per digit d: counts[d * tiles + t] becomes the exclusive prefix over t, in
+// place, and totals[d] the digit's count.
+constexpr int kScanItems = 8;  // consecutive counts per thread and round
+__global__ __launch_bounds__(256) void digit_scan_kernel(uint32_t* __restrict__ counts, int64_t tiles,
+                                                         uint32_t* __restrict__ totals) {
   __shared__ uint32_t s[256];
+  uint32_t* __restrict__ row = counts + static_cast<int64_t>(blockIdx.x) * tiles;
   uint32_t carry = 0;
-  for (int64_t base = 0; base < m; base += 256) {
-    const int64_t j = base + threadIdx.x;
-    const uint32_t v = j < m ? sums[j] : 0u;
+  for (int64_t base = 0; base < tiles; base += 256 * kScanItems) {
+    uint32_t v[kScanItems];
+    uint32_t local = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const int64_t j = base + static_cast<int64_t>(threadIdx.x) * kScanItems + i;
+      v[i] = j < tiles ? row[j] : 0u;
+      local += v[i];
+    }
     uint32_t ex;
-    const uint32_t total = block_exscan256(s, v, &ex);
-    if (j < m) sums[j] = carry + ex;
+    const uint32_t total = block_exscan256(s, local, &ex);
+    uint32_t run = carry + ex;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const int64_t j = base + static_cast<int64_t>(threadIdx.x) * kScanItems + i;
+      if (j < tiles) row[j] = run;
+      run += v[i];
+    }
     carry += total;
   }
-}
-
-__global__ __launch_bounds__(256) void scan_down_kernel(uint32_t* __restrict__ x, int64_t n,
-                                                        const uint32_t* __restrict__ sums) {
-  __shared__ uint32_t s[256];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk;
-  constexpr int per = kScanChunk / 256;  // consecutive elements per thread
-  uint32_t v[per];
-  uint32_t local = 0;
-#pragma unroll
-  for (int i = 0; i < per; ++i) {
-    const int64_t j = base + threadIdx.x * per + i;
-    v[i] = j < n ? x[j] : 0u;
-    local += v[i];
-  }
-  uint32_t ex;
-  block_exscan256(s, local, &ex);
-  uint32_t run = sums[blockIdx.x] + ex;
-#pragma unroll
-  for (int i = 0; i < per; ++i) {
-    const int64_t j = base + threadIdx.x * per + i;
-    if (j < n) x[j] = run;
-    run += v[i];
-  }
-}
-
-// Small scans (the sorts of up to 128K keys): one block of 1024 threads, each thread
-// owning a run of consecutive elements -- one launch instead of three.
-constexpr int kScanOneThreads = 1024;
-constexpr int64_t kScanOneMax = 8192;
-
-__global__ __launch_bounds__(kScanOneThreads) void scan_one_block_kernel(uint32_t* __restrict__ x, int64_t n) {
-  __shared__ uint32_t wsum[kScanOneThreads / 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t per = (n + kScanOneThreads - 1) / kScanOneThreads;
-  const int64_t b = threadIdx.x * per;
-  const int64_t e = b + per < n ? b + per : n;
-  uint32_t local = 0;
-  for (int64_t j = b; j < e; ++j) local += x[j];
-  // inclusive scan within the wave, then over the wave totals
-  uint32_t incl = local;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t t = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += t;
-  }
-  if (lane == 63) wsum[wv] = incl;
-  __syncthreads();
-  if (wv == 0) {
-    uint32_t w = lane < kScanOneThreads / 64 ? wsum[lane] : 0u;
-#pragma unroll
-    for (int off = 1; off < kScanOneThreads / 64; off <<= 1) {
-      const uint32_t t = __shfl_up(w, off, 64);
-      if (lane >= off) w += t;
-    }
-    if (lane < kScanOneThreads / 64) wsum[lane] = w;  // inclusive over waves
-  }
-  __syncthreads();
-  uint32_t run = (wv > 0 ? wsum[wv - 1] : 0u) + incl - local;
-  for (int64_t j = b; j < e; ++j) {
-    const uint32_t v = x[j];
-    x[j] = run;
-    run += v;
-  }
-}
-
-static hipError_t exclusive_scan(uint32_t* x, int64_t n, uint32_t* sums, hipStream_t st) {
-  if (n <= kScanOneMax) {
-    hipLaunchKernelGGL(scan_one_block_kernel, dim3(1), dim3(kScanOneThreads), 0, st, x, n);
-    return hipGetLastError();
-  }
-  const int64_t chunks = (n + kScanChunk - 1) / kScanChunk;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, st, x, n, sums);
-  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, st, sums, chunks);
-  hipLaunchKernelGGL(scan_down_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, st, x, n, sums);
-  return hipGetLastError();
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // ---- stable scatter through LDS
@@ -174,8 +99,8 @@ static hipError_t exclusive_scan(uint32_t* x, int64_t n, uint32_t* sums, hipStre
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift, int64_t tiles,
-    const uint32_t* __restrict__ start, const uint32_t* __restrict__ tile_counts, K* __restrict__ kout,
-    uint32_t* __restrict__ vout) {
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ totals, const uint32_t* __restrict__ tile_counts,
+    K* __restrict__ kout, uint32_t* __restrict__ vout) {
   __shared__ K skey[kTileKeys];
   __shared__ uint32_t sval[kTileKeys];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
@@ -193,7 +118,9 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   block_exscan256(scan_tmp, tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x], &ex);
   loc[threadIdx.x] = ex;
   run[threadIdx.x] = ex;
-  gpos[threadIdx.x] = start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
+  uint32_t dbase;  // this digit's first output position: the counts of the lower digits
+  block_exscan256(scan_tmp, totals[threadIdx.x], &dbase);
+  gpos[threadIdx.x] = dbase + start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
   __syncthreads();
 
   // all of this thread's keys and values are loaded up front, so their loads are in
@@ -252,8 +179,7 @@ static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 size_t radix_sort_workspace(int64_t n, size_t key_bytes) {
   const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
   const int64_t m = tiles * 256;
-  const int64_t chunks = (m + kScanChunk - 1) / kScanChunk;
-  return al(n * key_bytes) + al(n * 4) + 2 * al(m * 4) + al(chunks * 4);
+  return al(n * key_bytes) + al(n * 4) + 2 * al(m * 4) + al(256 * 4);
 }
 
 template <typename K>
@@ -271,7 +197,7 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
   w += al(m * 4);
   uint32_t* tile_counts = reinterpret_cast<uint32_t*>(w);
   w += al(m * 4);
-  uint32_t* sums = reinterpret_cast<uint32_t*>(w);
+  uint32_t* totals = reinterpret_cast<uint32_t*>(w);  // [256] per-digit counts
   const int passes = (end_bit - begin_bit + 7) / 8;
   const K* ksrc = keys_in;
   const uint32_t* vsrc = vals_in;  // null: the pass numbers the input positions
@@ -283,11 +209,10 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
     uint32_t* vdst = to_out ? vals_out : vtmp;
     hipLaunchKernelGGL((digit_hist_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
                        ksrc, n, shift, tiles, counts, tile_counts);
-    hipError_t e = exclusive_scan(counts, m, sums, st);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, st, counts, tiles, totals);
     hipLaunchKernelGGL((digit_scatter_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
-                       ksrc, vsrc, n, shift, tiles, counts, tile_counts, kdst, vdst);
-    e = hipGetLastError();
+                       ksrc, vsrc, n, shift, tiles, counts, totals, tile_counts, kdst, vdst);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ksrc = kdst;
     vsrc = vdst;
