@@ -11,9 +11,9 @@
 //      records the digit's total
 //   3. digit_scatter: the digit bases (a scan of the 256 totals, per block), local digit
 //      starts from the tile's own counts (kept by step 1);
-//      each tile walks its keys in order, 256 at a time; a key's rank
-//      among equal digits is (earlier chunks) + (earlier waves of this chunk) +
-//      (earlier lanes of its wave, from 8 ballots over the digit bits).  Keys are
+//      each wave ranks its own contiguous run of the tile's keys in order; a key's rank
+//      among equal digits is (earlier waves' counts) + (earlier chunks of its wave) +
+//      (earlier lanes of its chunk, from 8 ballots over the digit bits).  Keys are
 //      placed in LDS at their tile-local rank, then written out run by run.
 #include "aknn_common.h"
 #include "aknn_internal.h"
@@ -105,8 +105,7 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   __shared__ uint32_t sval[kTileKeys];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
   __shared__ uint32_t gpos[256];       // global start of each digit's run of this tile
-  __shared__ uint32_t run[256];        // next local slot of each digit
-  __shared__ uint32_t wcnt[4][256];    // per-wave digit counts of the current chunk
+  __shared__ uint32_t wcnt[4][256];    // per-wave digit counts, then each wave's digit starts
   __shared__ uint32_t scan_tmp[256];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -117,53 +116,66 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   uint32_t ex;
   block_exscan256(scan_tmp, tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x], &ex);
   loc[threadIdx.x] = ex;
-  run[threadIdx.x] = ex;
   uint32_t dbase;  // this digit's first output position: the counts of the lower digits
   block_exscan256(scan_tmp, totals[threadIdx.x], &dbase);
   gpos[threadIdx.x] = dbase + start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
-  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
 
-  // all of this thread's keys and values are loaded up front, so their loads are in
-  // flight together rather than one chunk at a time behind the barriers
+  // Each wave ranks its own contiguous run of kSortItems * 64 keys of the tile, in input
+  // order: a key's rank within its wave's run for its digit is the wave's count of that
+  // digit so far (in LDS; the wave's LDS operations complete in program order) plus the
+  // earlier lanes of its chunk with the same digit (8 ballots over the digit bits).  No
+  // block barrier inside the run.  All the run's keys and values are loaded up front.
+  const int wbase = wave * (kSortItems * 64);
   K kreg[kSortItems];
-  uint32_t vreg[kSortItems];
+  uint32_t vreg[kSortItems], lrank[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-    const int jj = i * kSortThreads + threadIdx.x;
+    const int jj = wbase + i * 64 + lane;
     const bool ok = jj < tile_n;
     kreg[i] = ok ? kin[base + jj] : K(0);
     vreg[i] = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
   }
+  __syncthreads();  // loc, gpos and the zeroed counts are visible
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
-    const int jj = i * kSortThreads + threadIdx.x;
-    const bool ok = jj < tile_n;
-    const K key = kreg[i];
-    const uint32_t val = vreg[i];
-    const uint32_t d = ok ? static_cast<uint32_t>((key >> shift) & 255u) : 256u;
+    const bool ok = wbase + i * 64 + lane < tile_n;
+    const uint32_t d = ok ? static_cast<uint32_t>((kreg[i] >> shift) & 255u) : 256u;
     unsigned long long same = __ballot(ok);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const unsigned long long m = __ballot(ok && ((d >> b) & 1u));
       same &= ((d >> b) & 1u) ? m : ~m;
     }
-    const uint32_t rank_w = static_cast<uint32_t>(__popcll(same & lt));
-    if (ok && rank_w == 0) wcnt[wave][d] = static_cast<uint32_t>(__popcll(same));
-    __syncthreads();
-    if (ok) {
-      uint32_t off = run[d] + rank_w;
-      for (int w = 0; w < wave; ++w) off += wcnt[w][d];
-      skey[off] = key;
-      sval[off] = val;
-    }
-    __syncthreads();
-    run[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
-    __syncthreads();
+    const uint32_t rank_in = static_cast<uint32_t>(__popcll(same & lt));
+    const uint32_t prev = ok ? wcnt[wave][d] : 0u;
+    lrank[i] = prev + rank_in;
+    if (ok && rank_in == 0) wcnt[wave][d] = prev + static_cast<uint32_t>(__popcll(same));
+    __builtin_amdgcn_wave_barrier();  // the next chunk's reads follow this chunk's update
   }
+  __syncthreads();
+  {  // digit t: each wave's start = the tile's digit start + the earlier waves' counts
+    uint32_t acc = loc[threadIdx.x];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t c = wcnt[w][threadIdx.x];
+      wcnt[w][threadIdx.x] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    if (wbase + i * 64 + lane < tile_n) {
+      const uint32_t d = static_cast<uint32_t>((kreg[i] >> shift) & 255u);
+      const uint32_t off = wcnt[wave][d] + lrank[i];
+      skey[off] = kreg[i];
+      sval[off] = vreg[i];
+    }
+  }
+  __syncthreads();
   // coalesced write-out of the locally sorted tile
   for (int jj = threadIdx.x; jj < tile_n; jj += kSortThreads) {
     const K key = skey[jj];
