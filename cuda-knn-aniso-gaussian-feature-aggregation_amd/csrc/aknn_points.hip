@@ -647,6 +647,22 @@ __global__ void csr_keys_kernel(const int64_t* __restrict__ idxs, Clouds cl, int
   keys[e] = static_cast<uint32_t>(cl.p_begin(b) + idxs[e]);
 }
 
+// Four entries per thread (K divisible by 4, so the four share a query row): one
+// 32-bit division and one cloud lookup for the four, 16-byte loads and store.  The
+// entry count is below 2^31 (the argument check), and idxs is 16-byte aligned.
+__global__ void csr_keys4_kernel(const int64_t* __restrict__ idxs, Clouds cl, uint32_t K,
+                                 uint32_t entries, uint32_t* __restrict__ keys) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t e = 4 * t;
+  if (e >= entries) return;
+  const int64_t qrow = e / K;
+  const uint32_t pb = static_cast<uint32_t>(cl.p_begin(cl.cloud_of_query(qrow)));
+  const longlong2 a = *reinterpret_cast<const longlong2*>(idxs + e);
+  const longlong2 c = *reinterpret_cast<const longlong2*>(idxs + e + 2);
+  *reinterpret_cast<uint4*>(keys + e) = make_uint4(pb + static_cast<uint32_t>(a.x), pb + static_cast<uint32_t>(a.y),
+                                                   pb + static_cast<uint32_t>(c.x), pb + static_cast<uint32_t>(c.y));
+}
+
 // off[i] = first position of point row i in the sorted keys (lower bound), i in [0, rows].
 __global__ void csr_offsets_kernel(const uint32_t* __restrict__ sorted, int64_t entries,
                                    int64_t rows, int32_t* __restrict__ off) {
@@ -686,8 +702,14 @@ hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, 
   void* tmp = base + 2 * align256(entries * 4);
   if (entries > 0) {
     const int bs = 256;
-    hipLaunchKernelGGL(csr_keys_kernel, dim3(static_cast<unsigned>((entries + bs - 1) / bs)), dim3(bs),
-                       0, st, idxs, cl, K, entries, keys_in);
+    if (K % 4 == 0 && reinterpret_cast<uintptr_t>(idxs) % 16 == 0) {
+      const int64_t threads = entries / 4;
+      hipLaunchKernelGGL(csr_keys4_kernel, dim3(static_cast<unsigned>((threads + bs - 1) / bs)), dim3(bs), 0, st,
+                         idxs, cl, static_cast<uint32_t>(K), static_cast<uint32_t>(entries), keys_in);
+    } else {
+      hipLaunchKernelGGL(csr_keys_kernel, dim3(static_cast<unsigned>((entries + bs - 1) / bs)), dim3(bs),
+                         0, st, idxs, cl, K, entries, keys_in);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     err = radix_sort_pairs<uint32_t>(keys_in, keys_out, nullptr, reinterpret_cast<uint32_t*>(csr_ent),
