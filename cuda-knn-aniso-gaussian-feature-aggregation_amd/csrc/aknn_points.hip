@@ -590,6 +590,39 @@ hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
 // dq[q] = sum over k of entry (q, k)'s share, in slot order: one thread per query.  When
 // a query's 3K shares come in whole 16-byte vectors (FP32, K a multiple of 4) the
 // thread reads them as such, otherwise one value at a time.
+// K = 16, FP32: each wave stages its 64 queries' 12 KB of shares through LDS with
+// coalesced 16-byte loads, then each lane sums its own query in the same order as
+// contrib_reduce_kernel (same bits).
+__global__ __launch_bounds__(256) void contrib_reduce16_kernel(const float4* __restrict__ contrib, int64_t rows,
+                                                               float* __restrict__ dq) {
+  constexpr int kPer = 12;  // float4 per query: 16 entries x 3 values
+  __shared__ float4 buf[4][64 * kPer];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 256 + wave * 64;
+  const int64_t n4 = rows * kPer;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int64_t g = q0 * kPer + j * 64 + lane;
+    if (g < n4) buf[wave][j * 64 + lane] = contrib[g];
+  }
+  __syncthreads();
+  const int64_t q = q0 + lane;
+  if (q >= rows) return;
+  const float4* v = buf[wave] + lane * kPer;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < kPer; k += 3) {  // four entries = three vectors
+    const float4 a = v[k], b = v[k + 1], c = v[k + 2];
+    s0 += a.x; s1 += a.y; s2 += a.z;
+    s0 += a.w; s1 += b.x; s2 += b.y;
+    s0 += b.z; s1 += b.w; s2 += c.x;
+    s0 += c.y; s1 += c.z; s2 += c.w;
+  }
+  dq[q * 3 + 0] = s0;
+  dq[q * 3 + 1] = s1;
+  dq[q * 3 + 2] = s2;
+}
+
 template <typename T, bool VEC4>
 __global__ __launch_bounds__(256) void contrib_reduce_kernel(const T* __restrict__ contrib, int64_t rows, int K,
                                                              T* __restrict__ dq) {
@@ -624,7 +657,10 @@ hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, v
   if (rows == 0) return hipSuccess;
   const dim3 grid(static_cast<unsigned>((rows + 255) / 256)), block(256);
   const bool vec4 = dtype == AKNN_F32 && K % 4 == 0 && (reinterpret_cast<uintptr_t>(contrib) % 16) == 0;
-  if (dtype == AKNN_F32 && vec4)
+  if (dtype == AKNN_F32 && vec4 && K == 16)
+    hipLaunchKernelGGL(contrib_reduce16_kernel, grid, block, 0, st, static_cast<const float4*>(contrib), rows,
+                       static_cast<float*>(dq));
+  else if (dtype == AKNN_F32 && vec4)
     hipLaunchKernelGGL((contrib_reduce_kernel<float, true>), grid, block, 0, st,
                        static_cast<const float*>(contrib), rows, static_cast<int>(K), static_cast<float*>(dq));
   else if (dtype == AKNN_F32)
