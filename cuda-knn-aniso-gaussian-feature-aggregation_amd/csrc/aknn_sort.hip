@@ -5,7 +5,7 @@
 // workspace: no allocation, no memset, no host synchronisation, so the whole sort can
 // be captured in a graph.  Stable and deterministic: equal keys keep input order.
 //
-// Per pass over digit (key >> shift) & 255, with tiles of kTileKeys keys per block:
+// Per pass over digit (key >> shift) & 255, with tiles of tile_keys<K>() keys per block:
 //   1. digit_hist: per-tile digit counts (LDS), stored digit-major: counts[d * tiles + t]
 //   2. digit_scan: one block per digit scans that digit's tile counts (in place) and
 //      records the digit's total
@@ -21,14 +21,21 @@
 namespace aknn {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kTileKeys = kSortThreads * kSortItems;  // 4096
+// Keys per thread: 16 for 32-bit keys (4096-key tiles), 8 for 64-bit keys (2048-key
+// tiles).  The 64-bit sorts are the Morton orders, one key per query or point: a 1M-query
+// sort gets 512 blocks instead of 256, enough to give every CU more than one.
+template <typename K>
+constexpr int sort_items() { return sizeof(K) == 8 ? 8 : 16; }
+template <typename K>
+constexpr int tile_keys() { return kSortThreads * sort_items<K>(); }
 
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __restrict__ keys, int64_t n,
                                                                   int shift, int64_t tiles,
                                                                   uint32_t* __restrict__ counts,
                                                                   uint32_t* __restrict__ tile_counts) {
+  constexpr int kSortItems = sort_items<K>();
+  constexpr int kTileKeys = tile_keys<K>();
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -101,6 +108,8 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift, int64_t tiles,
     const uint32_t* __restrict__ start, const uint32_t* __restrict__ totals, const uint32_t* __restrict__ tile_counts,
     K* __restrict__ kout, uint32_t* __restrict__ vout) {
+  constexpr int kSortItems = sort_items<K>();
+  constexpr int kTileKeys = tile_keys<K>();
   __shared__ K skey[kTileKeys];
   __shared__ uint32_t sval[kTileKeys];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
@@ -189,7 +198,8 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t radix_sort_workspace(int64_t n, size_t key_bytes) {
-  const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
+  const int64_t tk = key_bytes == 8 ? tile_keys<uint64_t>() : tile_keys<uint32_t>();
+  const int64_t tiles = (n + tk - 1) / tk;
   const int64_t m = tiles * 256;
   return al(n * key_bytes) + al(n * 4) + 2 * al(m * 4) + al(256 * 4);
 }
@@ -198,6 +208,7 @@ template <typename K>
 hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
                             int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  constexpr int64_t kTileKeys = tile_keys<K>();
   const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
   const int64_t m = tiles * 256;
   char* w = static_cast<char*>(ws);
