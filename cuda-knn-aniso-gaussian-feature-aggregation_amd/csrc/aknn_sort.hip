@@ -36,14 +36,40 @@ __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(const K* __res
                                                                   uint32_t* __restrict__ tile_counts) {
   constexpr int kSortItems = sort_items<K>();
   constexpr int kTileKeys = tile_keys<K>();
+  constexpr int kVec = 16 / static_cast<int>(sizeof(K));  // keys per 16-byte load
+  constexpr int kLoads = kSortItems / kVec;
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kTileKeys;
+  // integer counts: the order of the additions does not matter
+  if (base + kTileKeys <= n && (reinterpret_cast<uintptr_t>(keys) % 16) == 0) {
+    // whole tile, 16-byte loads (coalesced: thread t takes vectors t, t + 256, ...), all
+    // issued before the first count
+    const uint4* kv = reinterpret_cast<const uint4*>(keys + base);
+    uint4 r[kLoads];
+#pragma unroll
+    for (int i = 0; i < kLoads; ++i) r[i] = kv[i * kSortThreads + threadIdx.x];
+#pragma unroll
+    for (int i = 0; i < kLoads; ++i) {
+      if constexpr (sizeof(K) == 4) {
+        atomicAdd(&h[(r[i].x >> shift) & 255u], 1u);
+        atomicAdd(&h[(r[i].y >> shift) & 255u], 1u);
+        atomicAdd(&h[(r[i].z >> shift) & 255u], 1u);
+        atomicAdd(&h[(r[i].w >> shift) & 255u], 1u);
+      } else {
+        const uint64_t k0 = static_cast<uint64_t>(r[i].x) | (static_cast<uint64_t>(r[i].y) << 32);
+        const uint64_t k1 = static_cast<uint64_t>(r[i].z) | (static_cast<uint64_t>(r[i].w) << 32);
+        atomicAdd(&h[(k0 >> shift) & 255u], 1u);
+        atomicAdd(&h[(k1 >> shift) & 255u], 1u);
+      }
+    }
+  } else {
 #pragma unroll 4
-  for (int i = 0; i < kSortItems; ++i) {
-    const int64_t j = base + i * kSortThreads + threadIdx.x;
-    if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);  // integer: order-independent
+    for (int i = 0; i < kSortItems; ++i) {
+      const int64_t j = base + i * kSortThreads + threadIdx.x;
+      if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);
+    }
   }
   __syncthreads();
   counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];       // scanned next

@@ -124,14 +124,14 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         # reference, csrc/backward.cu:93-94): no grad on f_out means no gradient at all.
         if grad_f_out is None:
             return (None,) * 7
-        if grad_w_out is None:
-            grad_w_out = torch.zeros_like(w_out)
+        if grad_w_out is None:  # never read: a broadcast zero instead of a Q x K fill
+            grad_w_out = w_out.new_zeros(()).expand_as(w_out)
         needs = tuple(ctx.needs_input_grad[:5])
         # w_out is detached: the backward never reads it, and an edge from the
         # second-order graph back into this node would only run it again on zeros.
         grad_q, grad_p, grad_f, grad_sigma, grad_R = KNNAnisotropicAggregateBackward.apply(
             grad_f_out.contiguous(),
-            grad_w_out.contiguous(),
+            grad_w_out,  # never read
             q,
             p,
             f,
@@ -237,7 +237,7 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
             None if grad_grad_q is None else grad_grad_q.contiguous(),
             grad_f_out.contiguous(),
-            grad_w_out.contiguous(),
+            grad_w_out,  # never read
             q,
             p,
             f,
