@@ -39,6 +39,17 @@ __device__ __forceinline__ void vstore(T* __restrict__ dst, const T (&s)[VEC]) {
   *reinterpret_cast<VecT<T, VEC>*>(dst) = x;
 }
 
+// Streaming store (non-temporal): outputs that nothing in the kernel reads again, so
+// they do not push the gathered point and feature rows out of L2.
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore_nt(T* __restrict__ dst, const T (&s)[VEC]) {
+  typedef T V __attribute__((ext_vector_type(VEC)));
+  V x;
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) x[i] = s[i];
+  __builtin_nontemporal_store(x, reinterpret_cast<V*>(dst));
+}
+
 // Broadcast of lane J of each 16-lane row (ds_swizzle, bit mode: and 0x10, or J).
 template <int J>
 __device__ __forceinline__ int row_get(int v) {
@@ -169,10 +180,10 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
           D = pair_key<T, AKNN_KEY_ANISO>(qx, qy, qz, pt.p0, pt.p1, pt.p2, pt.r[0], pt.r[1], pt.r[2],
                                           pt.r[3], pt.r[4], pt.r[5], pt.r[6], pt.r[7], pt.r[8], pt.s0,
                                           pt.s1, pt.s2);
-          if (row_ok && L.dists_out) static_cast<T*>(L.dists_out)[row * K + k] = D;
+          if (row_ok && L.dists_out) __builtin_nontemporal_store(D, static_cast<T*>(L.dists_out) + row * K + k);
         }
         w = gexp(-D);
-        if (row_ok && L.w_out) static_cast<T*>(L.w_out)[row * K + k] = w;
+        if (row_ok && L.w_out) __builtin_nontemporal_store(w, static_cast<T*>(L.w_out) + row * K + k);
       }
       coef = w;
     } else if constexpr (MODE == kBwd2F) {
@@ -275,7 +286,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
             for (int i = 0; i < VEC; ++i) cur[i] += acc[c][i];
             vstore<T, VEC>(out + row * F + ft, cur);
           } else {
-            vstore<T, VEC>(out + row * F + ft, acc[c]);
+            vstore_nt<T, VEC>(out + row * F + ft, acc[c]);
           }
         }
       }
