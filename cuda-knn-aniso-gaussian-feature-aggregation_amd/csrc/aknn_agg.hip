@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   for (int kb = 0; kb < K; kb += 16) {
     const int k = kb + r;
     const bool have = k < K;
-    const int n = have ? static_cast<int>(idxs[row * K + k]) : 0;
+    const int n = have ? static_cast<int>(__builtin_nontemporal_load(idxs + row * K + k)) : 0;  // read once
     T w = T(0), coef = T(0);
     Geo<T> geo{};
     Pt<T> pt{};
