@@ -698,11 +698,14 @@ struct Upstream2 {  // the second order's upstream beyond grad_grad_q (full seco
   bool any() const { return p || f || sigma || R; }
 };
 
+// Bytes of the point dot products of a broadcast upstream.
+size_t dots_bytes(const aknn_dims* d) { return align256(static_cast<size_t>(point_rows(d)) * 16 * elem_size(d)); }
+
 static int backward_common(const aknn_dims* d, uint32_t needs, bool second, const void* gg,
                            const Upstream2& up, const void* g,
                            const void* q, const void* p, const void* f, const void* sigma,
                            const void* R, const int64_t* idxs, const int32_t* csr_off,
-                           const int32_t* csr_ent, const aknn_cache* cache, void* ws,
+                           const int32_t* csr_ent, const aknn_cache* cache_in, void* ws,
                            size_t ws_bytes, void* dgf, void* dq, void* dp, void* df, void* dsigma,
                            void* dR, void* stream) {
   int rc = check_dims(d, true);
@@ -712,8 +715,18 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
   if (full && !(needs & AKNN_SECOND_FULL))
     return fail(AKNN_E_INVALID, "grad_grad_p / _f / _sigma / _R given: set AKNN_SECOND_FULL in needs");
   if (!full) needs &= ~AKNN_SECOND_FULL;
-  const bool via_contrib = contrib_path(d, needs);
-  if (via_contrib && (!ws || ws_bytes < aknn_backward_workspace_size(d, needs)))
+  const bool bcast = (needs & AKNN_GRAD_BROADCAST) != 0;
+  if (bcast && full)
+    return fail(AKNN_E_INVALID, "the full second order takes a materialised grad_f_out (no AKNN_GRAD_BROADCAST)");
+  // A forward over no query rows fills no cache (its orders are left unwritten): ignore it.
+  const aknn_cache* cache = query_rows(d) > 0 ? cache_in : nullptr;
+  const bool via_contrib = !bcast && contrib_path(d, needs);
+  // broadcast upstream: dq from point dot products, unless the query kernel runs for
+  // d grad_f_out anyway (it then forms dq alongside)
+  const bool dq_dots = bcast && (needs & AKNN_NEED_DQ) && !(second && (needs & AKNN_NEED_DGRAD_F_OUT)) &&
+                       query_rows(d) > 0;
+  const bool own_dots = dq_dots && !(cache && cache->point_dots);
+  if ((via_contrib || own_dots) && (!ws || ws_bytes < aknn_backward_workspace_size(d, needs)))
     return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
                 aknn_backward_workspace_size(d, needs));
   if (!g || !q || !p || !f || !sigma || !R || (query_rows(d) > 0 && !idxs))
@@ -724,7 +737,51 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
   if ((needs & AKNN_NEED_DF) && !df) return fail(AKNN_E_INVALID, "df requested but NULL");
   if ((needs & AKNN_NEED_DSIGMA) && !dsigma) return fail(AKNN_E_INVALID, "dsigma requested but NULL");
   if ((needs & AKNN_NEED_DR) && !dR) return fail(AKNN_E_INVALID, "dR requested but NULL");
-  const bool want_q = ((needs & AKNN_NEED_DQ) && !via_contrib) || (second && (needs & AKNN_NEED_DGRAD_F_OUT));
+  if (dq_dots) {
+    const void* dots = own_dots ? ws : cache->point_dots;
+    hipError_t e;
+    if (own_dots) {
+      aknn::PointLaunch D{};
+      D.dtype = d->dtype;
+      D.rows = point_rows(d);
+      D.F = d->F;
+      D.K = d->K;
+      D.f = f;
+      D.p = p;
+      D.sigma = sigma;
+      D.R = R;
+      D.grad_f_out = g;
+      D.df = (needs & AKNN_NEED_DF) ? df : nullptr;  // same vector width as the point kernel
+      D.g_bcast = true;
+      D.cl = clouds_of(d);
+      {
+        Span sp("point_dots", st);
+        e = aknn::launch_point_dots(D, ws, st);
+      }
+      if (e != hipSuccess) return hip_fail(e, "point_dots");
+    }
+    aknn::QueryLaunch L{};
+    L.dtype = d->dtype;
+    L.key = d->key;
+    L.cl = clouds_of(d);
+    L.rows = query_rows(d);
+    L.F = d->F;
+    L.K = d->K;
+    L.q = q; L.p = p; L.f = f; L.sigma = sigma; L.R = R;
+    L.idxs = idxs;
+    L.grad_grad_q = gg;
+    L.dq = dq;
+    L.order = cache ? cache->query_order : nullptr;
+    L.rec = cache ? cache->point_records : nullptr;
+    L.dots = dots;
+    {
+      Span sp(second ? "dq_dots2" : "dq_dots", st);
+      e = aknn::launch_dq_dots(L, second, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "dq_dots");
+  }
+  const bool want_q = ((needs & AKNN_NEED_DQ) && !via_contrib && !dq_dots) ||
+                      (second && (needs & AKNN_NEED_DGRAD_F_OUT));
   if (want_q && query_rows(d) > 0) {
     aknn::QueryLaunch L{};
     L.dtype = d->dtype;
@@ -744,6 +801,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.d_grad_f_out = (needs & AKNN_NEED_DGRAD_F_OUT) ? dgf : nullptr;
     L.order = cache ? cache->query_order : nullptr;
     L.rec = cache ? cache->point_records : nullptr;
+    L.g_bcast = bcast;
     hipError_t e;
     {
       Span sp(full ? "query_grad2_full" : (second ? "query_grad2" : "query_grad"), st);
@@ -792,6 +850,8 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.dR = (needs & AKNN_NEED_DR) ? dR : nullptr;
     L.order = cache ? cache->point_order : nullptr;
     L.dq_contrib = via_contrib ? ws : nullptr;
+    L.g_bcast = bcast;
+    L.cl = clouds_of(d);
     hipError_t e;
     {
       Span sp(full ? "point_grad2_full" : (second ? "point_grad2" : "point_grad"), st);
@@ -811,8 +871,36 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
 
 size_t aknn_backward_workspace_size(const aknn_dims* d, uint32_t needs) {
   if (check_dims(d, false) != AKNN_OK) return 0;
+  if (needs & AKNN_GRAD_BROADCAST) return (needs & AKNN_NEED_DQ) && query_rows(d) > 0 ? dots_bytes(d) : 0;
   if (!contrib_path(d, needs)) return 0;
   return align256(static_cast<size_t>(query_rows(d)) * d->K * 3 * elem_size(d));
+}
+
+int aknn_point_dots(const aknn_dims* d, const void* grad_rows, const void* f, const void* p, const void* sigma,
+                    const void* R, void* a, void* stream) {
+  int rc = check_dims(d, false);
+  if (rc) return rc;
+  if (point_rows(d) == 0) return AKNN_OK;
+  if (!grad_rows || !f || !p || !sigma || !R || !a) return fail(AKNN_E_INVALID, "null pointer");
+  aknn::PointLaunch D{};
+  D.dtype = d->dtype;
+  D.rows = point_rows(d);
+  D.F = d->F;
+  D.K = d->K;
+  D.f = f;
+  D.p = p;
+  D.sigma = sigma;
+  D.R = R;
+  D.grad_f_out = grad_rows;
+  D.g_bcast = true;
+  D.cl = clouds_of(d);
+  hipError_t e;
+  {
+    Span sp("point_dots", static_cast<hipStream_t>(stream));
+    e = aknn::launch_point_dots(D, a, static_cast<hipStream_t>(stream));
+  }
+  if (e != hipSuccess) return hip_fail(e, "point_dots");
+  return AKNN_OK;
 }
 
 int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* q,

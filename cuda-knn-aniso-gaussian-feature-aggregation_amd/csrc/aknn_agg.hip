@@ -127,10 +127,11 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     for (int i = 0; i < VEC; ++i) acc[c][i] = T(0);
   if constexpr (kBack) {
     const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
+    const int64_t grow = L.g_bcast ? b : row;  // broadcast upstream: one row per cloud
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
-      if (ft < F) vload<T, VEC>(gsl[c], g + row * F + ft);
+      if (ft < F) vload<T, VEC>(gsl[c], g + grow * F + ft);
       else
 #pragma unroll
         for (int i = 0; i < VEC; ++i) gsl[c][i] = T(0);
@@ -385,6 +386,137 @@ hipError_t launch_lookup_gather(const void* d1, const int64_t* i1, const int64_t
 }
 
 hipError_t launch_query_grad_full(const QueryLaunch& L, hipStream_t st) { return launch_q<kBwd2F>(L, st); }
+
+// ------------------------------------------------------------------ dq from point dots
+// Broadcast upstream: a = g . f[n] depends on the point alone and comes precomputed in
+// the dot records (L.dots: {p, R, s, a} per point row), so dq needs no feature rows at
+// all -- per query only its idxs row, the K dot records (L2-resident) and, in the second
+// order, grad_grad_q.  One 16-lane row per query, lane r on slot kb + r.  Each slot's
+// share is the point kernels' per-entry share expression, and the shares are added in
+// slot order starting from zero, as contrib_reduce adds them.
+//
+// A lane-per-record gather would touch a different cache line in every lane of every
+// load and keep the texture addresser busy for the whole kernel.  Instead each wave
+// loads its 64 slots' records cooperatively -- 16-byte chunks, the lanes of one record
+// adjacent, so each load covers 64 / CPR whole records -- into LDS, and each lane then
+// reads its own record from there.
+template <typename T>
+__device__ __forceinline__ void wave_records(const T* __restrict__ rec, int gp, T (*stage)[16], Pt<T>& o,
+                                             T& a) {
+  constexpr int kCpr = static_cast<int>(sizeof(T));  // 16-byte chunks per record (16 values)
+  constexpr int kVals = 16 / kCpr;                     // values per chunk
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
+  __syncthreads();  // the previous batch's reads of this block's stage are done
+#pragma unroll
+  for (int j = 0; j < kCpr; ++j) {
+    const int s = j * (64 / kCpr) + lane / kCpr;  // slot (= lane of the wave) whose record this lane fetches
+    const int c = lane % kCpr;
+    const int gs = __shfl(gp, s, 64);
+    const T* src = rec + static_cast<int64_t>(gs) * 16 + c * kVals;
+    if constexpr (sizeof(T) == 4) {
+      *reinterpret_cast<float4*>(&stage[wbase + s][c * kVals]) = *reinterpret_cast<const float4*>(src);
+    } else {
+      *reinterpret_cast<double2*>(&stage[wbase + s][c * kVals]) = *reinterpret_cast<const double2*>(src);
+    }
+  }
+  __syncthreads();
+  const T* v = stage[threadIdx.x];
+  o.p0 = v[0];
+  o.p1 = v[1];
+  o.p2 = v[2];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) o.r[i] = v[3 + i];
+  o.s0 = v[12];
+  o.s1 = v[13];
+  o.s2 = v[14];
+  a = v[15];
+}
+
+template <typename T, bool SECOND>
+__global__ __launch_bounds__(256) void dq_dots_kernel(QueryLaunch L) {
+  __shared__ __align__(16) T stage[256][16];  // one dot record per slot of the block's four waves
+  const T* __restrict__ q = static_cast<const T*>(L.q);
+  const T* __restrict__ dots = static_cast<const T*>(L.dots);
+  const int64_t* __restrict__ idxs = L.idxs;
+  const int K = static_cast<int>(L.K);
+
+  const int r = threadIdx.x & 15;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gi = blk * 16 + (threadIdx.x >> 4);
+  const bool row_ok = gi < L.rows;
+  const int64_t gc = row_ok ? gi : L.rows - 1;
+  const int64_t row = L.order ? static_cast<int64_t>(L.order[gc]) : gc;
+  const int64_t pb = L.cl.p_begin(L.cl.cloud_of_query(row));
+  const T qx = q[row * 3 + 0], qy = q[row * 3 + 1], qz = q[row * 3 + 2];
+  T hx = T(0), hy = T(0), hz = T(0);
+  if constexpr (SECOND) {
+    const T* __restrict__ hh = static_cast<const T*>(L.grad_grad_q);
+    hx = hh[row * 3 + 0];
+    hy = hh[row * 3 + 1];
+    hz = hh[row * 3 + 2];
+  }
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  for (int kb = 0; kb < K; kb += 16) {
+    const int k = kb + r;
+    const bool have = k < K;
+    const int n = have ? static_cast<int>(__builtin_nontemporal_load(idxs + row * K + k)) : 0;
+    const int gp = static_cast<int>(pb) + n;  // point rows < 2^31 (checked by the C ABI)
+    Pt<T> pt;
+    T a;
+    wave_records<T>(dots, gp, stage, pt, a);
+    const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
+    const T w = have ? geo.w : T(0);
+    const T aw = a * w;
+    T c0, c1, c2;
+    if constexpr (!SECOND) {
+      c0 = T(-2) * aw * geo.v0;
+      c1 = T(-2) * aw * geo.v1;
+      c2 = T(-2) * aw * geo.v2;
+    } else {
+      const T s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+      T hr[3];  // h . R_i
+#pragma unroll
+      for (int i = 0; i < 3; ++i) hr[i] = pt.r[3 * i] * hx + pt.r[3 * i + 1] * hy + pt.r[3 * i + 2] * hz;
+      const T m0 = pt.s0 * hr[0], m1 = pt.s1 * hr[1], m2 = pt.s2 * hr[2];
+      const T mh0 = pt.r[0] * m0 + pt.r[3] * m1 + pt.r[6] * m2;
+      const T mh1 = pt.r[1] * m0 + pt.r[4] * m1 + pt.r[7] * m2;
+      const T mh2 = pt.r[2] * m0 + pt.r[5] * m1 + pt.r[8] * m2;
+      const T t0 = T(2) * s * geo.v0 - mh0, t1 = T(2) * s * geo.v1 - mh1, t2 = T(2) * s * geo.v2 - mh2;
+      c0 = T(2) * aw * t0;
+      c1 = T(2) * aw * t1;
+      c2 = T(2) * aw * t2;
+    }
+    const int nk = K - kb;  // slots in this batch (uniform)
+    static_for<0, 16>([&](auto J) {
+      const T x0 = row_get<J.value>(c0), x1 = row_get<J.value>(c1), x2 = row_get<J.value>(c2);
+      if (J.value < nk) {
+        s0 += x0;
+        s1 += x1;
+        s2 += x2;
+      }
+    });
+  }
+  if (row_ok && r < 3) static_cast<T*>(L.dq)[row * 3 + r] = r == 0 ? s0 : (r == 1 ? s1 : s2);
+}
+
+template <typename T, bool SECOND>
+static void launch_dq_dots_t(const QueryLaunch& L, dim3 grid, dim3 block, hipStream_t st) {
+  hipLaunchKernelGGL((dq_dots_kernel<T, SECOND>), grid, block, 0, st, L);
+}
+
+hipError_t launch_dq_dots(const QueryLaunch& L, bool second_order, hipStream_t st) {
+  if (L.rows == 0) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((L.rows + 15) / 16)), block(256);
+  if (L.dtype == AKNN_F32) {
+    if (second_order) launch_dq_dots_t<float, true>(L, grid, block, st);
+    else launch_dq_dots_t<float, false>(L, grid, block, st);
+  } else {
+    if (second_order) launch_dq_dots_t<double, true>(L, grid, block, st);
+    else launch_dq_dots_t<double, false>(L, grid, block, st);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_t st) {
   return second_order ? launch_q<kBwd2>(L, st) : launch_q<kBwd1>(L, st);

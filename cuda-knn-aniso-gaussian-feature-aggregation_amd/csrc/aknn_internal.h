@@ -69,6 +69,8 @@ struct QueryLaunch {
   // (kept last: the argument block of the other modes is unchanged)
   const void *gg_p, *gg_sigma, *gg_R;  // full second order: upstream of dp, dsigma, dR
   bool accumulate;         // forward: add into f_out instead of overwriting it
+  bool g_bcast;            // grad_f_out holds one row per cloud ([B, F]) instead of per query
+  const void* dots;        // dq from point dot products: a[n] = grad_f_out[cloud(n)] . f[n]
 };
 
 // Point-side work over the CSR transpose of idxs.
@@ -87,6 +89,8 @@ struct PointLaunch {
   // (kept last: the argument block of the first- and second-order kernels is unchanged)
   const void *gg_p, *gg_f, *gg_sigma, *gg_R;  // full second order: upstream of dp, df, dsigma, dR
   bool second;              // second order (grad_grad_q and the gg_* may each be null)
+  bool g_bcast;             // grad_f_out holds one row per cloud ([B, F]) instead of per query
+  Clouds cl;                // cloud of a point row (broadcast upstream)
 };
 
 template <typename T>
@@ -102,6 +106,12 @@ hipError_t launch_query_grad(const QueryLaunch& L, bool second_order, hipStream_
 // full second order: d grad_f_out = sum_k w Phi f[n] (no dq)
 hipError_t launch_query_grad_full(const QueryLaunch& L, hipStream_t st);
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st);
+// broadcast upstream: a[n] = grad_f_out[cloud(n)] . f[n] for every point row (L.g_bcast),
+// in the association of the point kernels
+hipError_t launch_point_dots(const PointLaunch& L, void* a, hipStream_t st);
+// dq (first or second order) per query from the point dot products L.dots, summed in slot
+// order as the per-entry shares are
+hipError_t launch_dq_dots(const QueryLaunch& L, bool second_order, hipStream_t st);
 
 hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st);
 hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st);

@@ -59,10 +59,35 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Broadcast upstream (one grad_f_out row per cloud, BC): this lane's slice of the
+// point row's cloud row, and a = g . f[row] reduced across the row in the association
+// of the per-entry dot products (row_transpose_sum over 16 equal partials: the same
+// pairing tree as for any one entry), so a and every df term keep the bits of a
+// materialised upstream.
+template <typename T, int VEC, int CH>
+__device__ __forceinline__ T bcast_row_dot(const T* __restrict__ g, int64_t b, int64_t F, const int64_t (&fofs)[CH],
+                                           const bool (&fok)[CH], const T (&fsl)[CH][VEC], T (&gbc)[CH][VEC],
+                                           int r) {
+  T dot = T(0);
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(g + b * F + fofs[c]);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      gbc[c][i] = fok[c] ? x.v[i] : T(0);
+      dot += fsl[c][i] * gbc[c][i];
+    }
+  }
+  T part[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) part[j] = dot;
+  return row_transpose_sum<T>(part, r);
+}
+
 // G = 16-lane groups per point row (1: four rows per wave; 4: one row per wave, its
 // entries dealt to the groups in turn).  Fewer rows in flight keep the gathered query
-// rows of the rows being processed within L2.
-template <typename T, int VEC, int CH, int KEY, bool SECOND, int G>
+// rows of the rows being processed within L2.  BC: broadcast upstream (no row gathers).
+template <typename T, int VEC, int CH, int KEY, bool SECOND, int G, bool BC>
 __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const T* __restrict__ q = static_cast<const T*>(L.q);
   const T* __restrict__ f = static_cast<const T*>(L.f);
@@ -110,6 +135,9 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   T acc[15];  // dp[3], dsigma[3], dR[9]
 #pragma unroll
   for (int i = 0; i < 15; ++i) acc[i] = T(0);
+  T gbc[BC ? CH : 1][VEC];
+  T a_row = T(0);
+  if constexpr (BC) a_row = bcast_row_dot<T, VEC, CH>(g, L.cl.cloud_of_point(row), F, fofs, fok, fsl, gbc, r);
 
   for (int eb = e0; eb < e1; eb += 16 * G) {
     const int e = eb + 16 * gq + r;
@@ -131,35 +159,46 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     }
     const int qi32 = static_cast<int>(qrow);
 
-    // All 16 row gathers of the batch are issued back to back: no per-lane branch in
-    // this loop (lanes past F read a clamped in-row address and mask the value).
-    int qj[16];
     T cj[16];
-    pstatic_for<0, 16>([&](auto J) {
-      qj[J.value] = prow_get<J.value>(qi32);
-      cj[J.value] = prow_get<J.value>(coef);
-    });
-    PVec<T, VEC> xs[16][CH];
-    pstatic_for<0, 16>([&](auto J) {
-      const T* __restrict__ grow = g + static_cast<int64_t>(qj[J.value]) * F;
+    pstatic_for<0, 16>([&](auto J) { cj[J.value] = prow_get<J.value>(coef); });
+    T a;
+    if constexpr (BC) {
+      // every entry's upstream row is the cloud's row, already in registers
+      pstatic_for<0, 16>([&](auto J) {
 #pragma unroll
-      for (int c = 0; c < CH; ++c) xs[J.value][c] = *reinterpret_cast<const PVec<T, VEC>*>(grow + fofs[c]);
-    });
-    T part[16];
-    pstatic_for<0, 16>([&](auto J) {
-      T dot = T(0);
+        for (int c = 0; c < CH; ++c)
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
+          for (int i = 0; i < VEC; ++i) dfa[c][i] += cj[J.value] * gbc[c][i];
+      });
+      a = a_row;
+      (void)qi32;
+    } else {
+      // All 16 row gathers of the batch are issued back to back: no per-lane branch in
+      // this loop (lanes past F read a clamped in-row address and mask the value).
+      int qj[16];
+      pstatic_for<0, 16>([&](auto J) { qj[J.value] = prow_get<J.value>(qi32); });
+      PVec<T, VEC> xs[16][CH];
+      pstatic_for<0, 16>([&](auto J) {
+        const T* __restrict__ grow = g + static_cast<int64_t>(qj[J.value]) * F;
 #pragma unroll
-        for (int i = 0; i < VEC; ++i) {
-          const T x = fok[c] ? xs[J.value][c].v[i] : T(0);
-          dfa[c][i] += cj[J.value] * x;
-          dot += fsl[c][i] * x;
+        for (int c = 0; c < CH; ++c) xs[J.value][c] = *reinterpret_cast<const PVec<T, VEC>*>(grow + fofs[c]);
+      });
+      T part[16];
+      pstatic_for<0, 16>([&](auto J) {
+        T dot = T(0);
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+          for (int i = 0; i < VEC; ++i) {
+            const T x = fok[c] ? xs[J.value][c].v[i] : T(0);
+            dfa[c][i] += cj[J.value] * x;
+            dot += fsl[c][i] * x;
+          }
         }
-      }
-      part[J.value] = dot;
-    });
-    const T a = row_transpose_sum<T>(part, r);
+        part[J.value] = dot;
+      });
+      a = row_transpose_sum<T>(part, r);
+    }
 
     if (have) {
       const T u[3] = {geo.u0, geo.u1, geo.u2};
@@ -530,9 +569,12 @@ static int pick_vec_points(const PointLaunch& L) {
 
 constexpr int kPointGroups = 4;  // 16-lane groups per point row
 
-template <typename T, int VEC, int KEY, bool SECOND, bool FULL>
+// The point kernels always weight with the anisotropic D (the key mode only ranks), so
+// they are instantiated once for both key modes.
+template <typename T, int VEC, bool SECOND, bool FULL, bool BC>
 static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
   constexpr int G = kPointGroups;
+  constexpr int KEY = AKNN_KEY_ANISO;
   const int64_t per = 16 * VEC;
   const int64_t ch = (L.F + per - 1) / per;
   const int64_t rows_per_block = 16 / G;
@@ -544,45 +586,115 @@ static hipError_t launch_p_ch(const PointLaunch& L, hipStream_t st) {
     else if (ch <= 8) hipLaunchKernelGGL((point_kernel_full<T, VEC, 8, KEY, true, G, true>), grid, block, 0, st, L);
     else return hipErrorNotSupported;
   } else {
-    if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G>), grid, block, 0, st, L);
-    else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G>), grid, block, 0, st, L);
-    else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G>), grid, block, 0, st, L);
-    else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G>), grid, block, 0, st, L);
+    if (ch <= 1) hipLaunchKernelGGL((point_kernel<T, VEC, 1, KEY, SECOND, G, BC>), grid, block, 0, st, L);
+    else if (ch <= 2) hipLaunchKernelGGL((point_kernel<T, VEC, 2, KEY, SECOND, G, BC>), grid, block, 0, st, L);
+    else if (ch <= 4) hipLaunchKernelGGL((point_kernel<T, VEC, 4, KEY, SECOND, G, BC>), grid, block, 0, st, L);
+    else if (ch <= 8) hipLaunchKernelGGL((point_kernel<T, VEC, 8, KEY, SECOND, G, BC>), grid, block, 0, st, L);
     else return hipErrorNotSupported;
   }
   return hipGetLastError();
 }
 
-template <typename T, int KEY, bool SECOND, bool FULL>
+template <typename T, bool SECOND, bool FULL, bool BC>
 static hipError_t launch_p_vf(const PointLaunch& L, hipStream_t st) {
   const int v = pick_vec_points<T>(L);
   if constexpr (sizeof(T) == 4) {
-    if (v == 4) return launch_p_ch<T, 4, KEY, SECOND, FULL>(L, st);
+    if (v == 4) return launch_p_ch<T, 4, SECOND, FULL, BC>(L, st);
   }
-  if (v == 2) return launch_p_ch<T, 2, KEY, SECOND, FULL>(L, st);
-  return launch_p_ch<T, 1, KEY, SECOND, FULL>(L, st);
+  if (v == 2) return launch_p_ch<T, 2, SECOND, FULL, BC>(L, st);
+  return launch_p_ch<T, 1, SECOND, FULL, BC>(L, st);
 }
 
-template <typename T, int KEY, bool SECOND>
+template <typename T, bool SECOND>
 static hipError_t launch_p_vec(const PointLaunch& L, hipStream_t st) {
   if constexpr (SECOND) {
-    if (L.gg_p || L.gg_f || L.gg_sigma || L.gg_R) return launch_p_vf<T, KEY, true, true>(L, st);
+    if (L.gg_p || L.gg_f || L.gg_sigma || L.gg_R) {
+      if (L.g_bcast) return hipErrorInvalidValue;  // the full second order takes a materialised upstream
+      return launch_p_vf<T, true, true, false>(L, st);
+    }
   }
-  return launch_p_vf<T, KEY, SECOND, false>(L, st);
+  if (L.g_bcast) return launch_p_vf<T, SECOND, false, true>(L, st);
+  return launch_p_vf<T, SECOND, false, false>(L, st);
 }
 
 template <bool SECOND>
 static hipError_t launch_p(const PointLaunch& L, hipStream_t st) {
-  if (L.dtype == AKNN_F32)
-    return L.key == AKNN_KEY_EUCLIDEAN ? launch_p_vec<float, AKNN_KEY_EUCLIDEAN, SECOND>(L, st)
-                                       : launch_p_vec<float, AKNN_KEY_ANISO, SECOND>(L, st);
-  return L.key == AKNN_KEY_EUCLIDEAN ? launch_p_vec<double, AKNN_KEY_EUCLIDEAN, SECOND>(L, st)
-                                     : launch_p_vec<double, AKNN_KEY_ANISO, SECOND>(L, st);
+  if (L.dtype == AKNN_F32) return launch_p_vec<float, SECOND>(L, st);
+  return launch_p_vec<double, SECOND>(L, st);
 }
 
 hipError_t launch_point_grad(const PointLaunch& L, hipStream_t st) {
   if (L.rows == 0) return hipSuccess;
   return L.second ? launch_p<true>(L, st) : launch_p<false>(L, st);
+}
+
+// ------------------------------------------------------------------ point dot products
+// Dot records of a broadcast upstream: for every point row n the 16 values
+// {p0 p1 p2, R00..R22, s0 s1 s2, a} with a = grad_f_out[cloud(n)] . f[n].  16 lanes per
+// row in the point kernels' feature layout and reduction, so a has their bits; lane r
+// then writes value r of the record (one coalesced 16-value row).  dq_dots reads a
+// neighbour's geometry and dot in one record.
+template <typename T, int VEC, int CH>
+__global__ __launch_bounds__(256) void point_dots_kernel(PointLaunch L, T* __restrict__ out) {
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const T* __restrict__ g = static_cast<const T*>(L.grad_f_out);
+  const int64_t F = L.F;
+  const int r = threadIdx.x & 15;
+  const int64_t gi = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const bool row_ok = gi < L.rows;
+  const int64_t row = row_ok ? gi : L.rows - 1;
+  T fsl[CH][VEC], gbc[CH][VEC];
+  int64_t fofs[CH];
+  bool fok[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
+    fok[c] = ft < F;
+    fofs[c] = fok[c] ? ft : 0;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) fsl[c][i] = T(0);
+    if (ft < F) {
+      const PVec<T, VEC> x = *reinterpret_cast<const PVec<T, VEC>*>(f + row * F + ft);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) fsl[c][i] = x.v[i];
+    }
+  }
+  const T a = bcast_row_dot<T, VEC, CH>(g, L.cl.cloud_of_point(row), F, fofs, fok, fsl, gbc, r);
+  T val;
+  if (r < 3) val = static_cast<const T*>(L.p)[row * 3 + r];
+  else if (r < 12) val = static_cast<const T*>(L.R)[row * 9 + (r - 3)];
+  else if (r < 15) val = static_cast<const T*>(L.sigma)[row * 3 + (r - 12)];
+  else val = a;
+  if (row_ok) out[row * 16 + r] = val;
+}
+
+template <typename T, int VEC>
+static hipError_t launch_dots_ch(const PointLaunch& L, void* a, hipStream_t st) {
+  const int64_t per = 16 * VEC;
+  const int64_t ch = (L.F + per - 1) / per;
+  const dim3 grid(static_cast<unsigned>((L.rows + 15) / 16)), block(256);
+  T* out = static_cast<T*>(a);
+  if (ch <= 1) hipLaunchKernelGGL((point_dots_kernel<T, VEC, 1>), grid, block, 0, st, L, out);
+  else if (ch <= 2) hipLaunchKernelGGL((point_dots_kernel<T, VEC, 2>), grid, block, 0, st, L, out);
+  else if (ch <= 4) hipLaunchKernelGGL((point_dots_kernel<T, VEC, 4>), grid, block, 0, st, L, out);
+  else if (ch <= 8) hipLaunchKernelGGL((point_dots_kernel<T, VEC, 8>), grid, block, 0, st, L, out);
+  else return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_dots_t(const PointLaunch& L, void* a, hipStream_t st) {
+  const int v = pick_vec_points<T>(L);  // the point kernels' choice for the same pointers
+  if constexpr (sizeof(T) == 4) {
+    if (v == 4) return launch_dots_ch<T, 4>(L, a, st);
+  }
+  if (v == 2) return launch_dots_ch<T, 2>(L, a, st);
+  return launch_dots_ch<T, 1>(L, a, st);
+}
+
+hipError_t launch_point_dots(const PointLaunch& L, void* a, hipStream_t st) {
+  if (L.rows == 0) return hipSuccess;
+  return L.dtype == AKNN_F32 ? launch_dots_t<float>(L, a, st) : launch_dots_t<double>(L, a, st);
 }
 
 // ------------------------------------------------------------------ dq from contributions

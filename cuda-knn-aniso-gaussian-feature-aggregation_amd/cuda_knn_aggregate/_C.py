@@ -30,6 +30,9 @@ __all__ = [
     "build_index",
     "make_dims",
     "new_cache",
+    "broadcast_rows",
+    "upstream",
+    "point_dots",
 ]
 
 _DTYPES = {torch.float32: _lib.AKNN_F32, torch.float64: _lib.AKNN_F64}
@@ -140,11 +143,45 @@ def new_cache(B, Q, P, dtype, device):
 
 
 def _cache_struct(cache):
+    """``aknn_cache`` from (query_order, point_order, point_records[, point_dots])."""
     if cache is None:
         return None
     s = _lib.Cache()
-    s.query_order, s.point_order, s.point_records = (t.data_ptr() if t is not None else None for t in cache)
+    ptrs = [t.data_ptr() if t is not None else None for t in cache] + [None] * (4 - len(cache))
+    s.query_order, s.point_order, s.point_records, s.point_dots = ptrs
     return s
+
+
+# ----------------------------------------------------------------------------- upstream
+def broadcast_rows(g, B: int, Q: int, F: int):
+    """``[B, F]`` rows of an upstream gradient ``g`` [B, Q, F] that repeats one row along the
+    query axis as a stride-0 view (``v.expand_as(f_out)``, as in the Eikonal read-out), or
+    ``None`` for any other layout.  The reference materialises such an upstream with
+    ``.contiguous()`` (cuda_knn_aggregate/__init__.py:107); the library takes its rows
+    instead (``AKNN_GRAD_BROADCAST``) with the same results."""
+    if not isinstance(g, torch.Tensor) or g.dim() != 3 or tuple(g.shape) != (B, Q, F) or Q < 1:
+        return None
+    if g.stride(1) != 0 or (g.stride(2) != 1 and F > 1):
+        return None
+    return g[:, 0, :].contiguous()
+
+
+def upstream(g):
+    """An upstream gradient as the backward entry points take it: broadcast views stay
+    views (their rows are read as such), anything else is made contiguous."""
+    if g is None:
+        return None
+    if g.dim() == 3 and broadcast_rows(g, *g.shape) is not None:
+        return g
+    return g.contiguous()
+
+
+def _gcols(g, rows, s0, s1):
+    """Columns [s0, s1) of an upstream gradient, keeping a broadcast upstream broadcast."""
+    if rows is None:
+        return _cols(g, s0, s1)
+    B, Q = g.shape[0], g.shape[1]
+    return rows[:, None, s0:s1].expand(B, Q, s1 - s0)
 
 
 def _cp(s):
@@ -205,7 +242,7 @@ def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tun
         f_out[..., s0:s1] = first
         for s0, s1 in rest:
             f_out[..., s0:s1] = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, idxs, K,
-                                                                        key=key)[0]
+                                                                        key=key, check=False)[0]
         return [f_out, w_out, dists, idxs]
     dev = q.device
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
@@ -228,12 +265,14 @@ def knn_aggregate_aniso_forward(q, p, f, sigma, R, K, *, key: str = "aniso", tun
     return [f_out, w_out, dists, idxs]
 
 
-def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, key: str = "aniso"):
+def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, key: str = "aniso",
+                                            check: bool = True):
     """Aggregation given neighbour indices -> ``[f_out, w_out, dists]``.
 
     Reference: ``knn_precomputed_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:76-99,
     csrc/forward.cu:186-230,398-450).  ``knn_idxs`` [B,Q,K] int64 must index [0, P);
-    it is range-checked here (one device reduction) because the kernel trusts it.
+    it is range-checked here (one device reduction and a host synchronisation) because
+    the kernel trusts it; ``check=False`` skips that for indices known to be in range.
     """
     for t, n in zip((q, p, f, sigma, R, knn_idxs), ("q", "p", "f", "sigma", "R", "knn_idxs")):
         _check_input(t, n)
@@ -247,8 +286,11 @@ def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, k
     slabs = _feature_slabs(F, dt)
     if slabs is not None:
         f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
+        if check:
+            _check_range(knn_idxs, P, "knn_idxs")
         for i, (s0, s1) in enumerate(slabs):
-            o = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_idxs, K, key=key)
+            o = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_idxs, K, key=key,
+                                                        check=False)
             f_out[..., s0:s1] = o[0]
             if i == 0:
                 w_out, dists = o[1], o[2]
@@ -259,9 +301,8 @@ def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, k
     dists = torch.empty(B, Q, K, dtype=dt, device=dev)
     if B * Q == 0:
         return [f_out, w_out, dists]
-    lo, hi = torch.aminmax(knn_idxs)
-    if int(lo) < 0 or int(hi) >= P:
-        raise RuntimeError(f"knn_idxs must lie in [0, {P}) (got [{int(lo)}, {int(hi)}])")
+    if check:
+        _check_range(knn_idxs, P, "knn_idxs")
     lib = _lib.load()
     dims = make_dims(B, Q, P, F, K, dt, key)
     with torch.cuda.device(dev):
@@ -272,7 +313,7 @@ def knn_precomputed_aggregate_aniso_forward(q, p, f, sigma, R, knn_idxs, K, *, k
     return [f_out, w_out, dists]
 
 
-def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
+def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K, *, check: bool = True):
     """Table-lookup forward -> ``[f_out, w_out, dists, idxs]``.
 
     Reference: ``knn_lookup_aggregate_aniso_forward`` (csrc/knn_aggregate.cpp:50-73,
@@ -280,7 +321,8 @@ def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
     index at the minimum; 0 unless that minimum is below 1e10) picks row ``n1`` of
     ``knn_table`` [P, >=K] (int64, shared by the batches); its first K entries are the
     neighbours, in table order, weighted with the anisotropic Gaussians.  The table is
-    range-checked here (one device reduction) because the kernels trust it.
+    range-checked here (one device reduction and a host synchronisation) because the
+    kernels trust it; ``check=False`` skips that for a table known to be in range.
     """
     for t, n in zip((q, p, f, sigma, R, knn_table), ("q", "p", "f", "sigma", "R", "knn_table")):
         _check_input(t, n)
@@ -297,10 +339,12 @@ def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
     if slabs is not None:  # one lookup, then the other column ranges on its neighbours
         (s0, s1), rest = slabs[0], slabs[1:]
         f_out = torch.empty(B, Q, F, dtype=dt, device=q.device)
-        first, w_out, dists, idxs = knn_lookup_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_table, K)
+        first, w_out, dists, idxs = knn_lookup_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, knn_table, K,
+                                                                       check=check)
         f_out[..., s0:s1] = first
         for s0, s1 in rest:
-            f_out[..., s0:s1] = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, idxs, K)[0]
+            f_out[..., s0:s1] = knn_precomputed_aggregate_aniso_forward(q, p, _cols(f, s0, s1), sigma, R, idxs, K,
+                                                                        check=False)[0]
         return [f_out, w_out, dists, idxs]
     dev = q.device
     f_out = torch.empty(B, Q, F, dtype=dt, device=dev)
@@ -309,9 +353,8 @@ def knn_lookup_aggregate_aniso_forward(q, p, f, sigma, R, knn_table, K):
     idxs = torch.empty(B, Q, K, dtype=torch.int64, device=dev)
     if B * Q == 0:
         return [f_out, w_out, dists, idxs]
-    lo, hi = torch.aminmax(knn_table[:, :K])
-    if int(lo) < 0 or int(hi) >= P:
-        raise RuntimeError(f"knn_table entries must lie in [0, {P}) (got [{int(lo)}, {int(hi)}])")
+    if check:
+        _check_range(knn_table[:, :K], P, "knn_table entries")
     lib = _lib.load()
     dims = make_dims(B, Q, P, F, K, dt)
     ws_bytes = lib.aknn_lookup_workspace_size(ctypes.byref(dims))
@@ -434,6 +477,39 @@ def knn_aniso(q, p, sigma, R, K, *, key: str = "aniso", tune=None):
     return [dists, idxs]
 
 
+def point_dots(rows, f, p, sigma, R, *, offsets=None):
+    """Dot records of a broadcast upstream: ``[n, 16]`` = {p, R, sigma, a} per point row
+    with ``a[n] = rows[cloud(n)] . f[n]`` (``rows`` [B, F]; ``f`` [B, P, F] and the point
+    tensors of the same batch, or packed ragged rows [sum P_i, ...] with ``offsets`` =
+    (q_off, p_off, q_off_host, p_off_host)).  ``a`` has the bits the backward kernels
+    form.  For the fourth entry of an ``aknn_cache`` tuple: a caller with a fixed
+    read-out and fixed points makes them once."""
+    for t, n in zip((rows, f, p, sigma, R), ("rows", "f", "p", "sigma", "R")):
+        _check_input(t, n)
+    dt = _check_float((rows, f, p, sigma, R), ("rows", "f", "p", "sigma", "R"))
+    if offsets is None:
+        if f.dim() != 3 or rows.dim() != 2 or tuple(rows.shape) != (f.shape[0], f.shape[2]):
+            raise RuntimeError("point_dots: rows must be [B, F] for f [B, P, F]")
+        B, P, F = f.shape
+        dims = make_dims(B, 1, P, F, 1, dt)
+        n = B * P
+    else:
+        B = int(offsets[2].numel()) - 1
+        if f.dim() != 2 or rows.dim() != 2 or tuple(rows.shape) != (B, f.shape[1]):
+            raise RuntimeError("point_dots: rows must be [B, F] for packed f [sum P_i, F]")
+        F = f.shape[1]
+        dims = make_dims(B, int(offsets[2][-1]), int(f.shape[0]), F, 1, dt, offsets=offsets)
+        n = int(f.shape[0])
+    if p.numel() != 3 * n or sigma.numel() != 3 * n or R.numel() != 9 * n:
+        raise RuntimeError("point_dots: p, sigma, R must hold 3, 3 and 9 values per point row")
+    out = torch.empty(n, 16, dtype=dt, device=f.device)
+    lib = _lib.load()
+    with torch.cuda.device(f.device):
+        _lib.check(lib.aknn_point_dots(ctypes.byref(dims), rows.data_ptr(), f.data_ptr(), p.data_ptr(),
+                                       sigma.data_ptr(), R.data_ptr(), out.data_ptr(), _stream(f.device)))
+    return out
+
+
 # ----------------------------------------------------------------------------- CSR
 def build_csr(idxs: torch.Tensor, P: int, key: str = "aniso"):
     """Point-side transpose of ``idxs`` [B,Q,K]: ``(offsets int32 [B*P+1], entries int32
@@ -476,6 +552,31 @@ _NEED_BITS = {
 }
 
 
+def _check_range(idx: torch.Tensor, P: int, name: str) -> None:
+    """Indices in [0, P): one device reduction and a host synchronisation (skip it with
+    ``check=False`` where the indices are known good, e.g. under graph capture)."""
+    if idx.numel() == 0:
+        return
+    lo, hi = torch.aminmax(idx)
+    if int(lo) < 0 or int(hi) >= P:
+        raise RuntimeError(f"{name} must lie in [0, {P}) (got [{int(lo)}, {int(hi)}])")
+
+
+def _bcast_of(g, q, f):
+    """Rows of a broadcast upstream for the batch of q and f, or None."""
+    if not isinstance(g, torch.Tensor) or q.dim() != 3 or f.dim() != 3:
+        return None
+    return broadcast_rows(g, q.shape[0], q.shape[1], f.shape[2])
+
+
+def _no_queries(names, mask, B, Q, P, F, dt, dev):
+    """Gradients of a batch without query rows: nothing reaches the points (zeros) and
+    the query-shaped outputs are empty.  No kernel runs."""
+    shapes = {"dgf": (B, Q, F), "dq": (B, Q, 3), "dp": (B, P, 3), "df": (B, P, F), "dsigma": (B, P, 3),
+              "dR": (B, P, 3, 3)}
+    return [torch.zeros(shapes[n], dtype=dt, device=dev) if mask & _NEED_BITS[n] else None for n in names]
+
+
 def _bwd_common(q, p, f, sigma, R, k_idxs, grads, gnames):
     for t, n in zip((q, p, f, sigma, R, k_idxs) + tuple(grads), ("q", "p", "f", "sigma", "R", "k_idxs") + tuple(gnames)):
         _check_input(t, n)
@@ -488,7 +589,7 @@ def _bwd_common(q, p, f, sigma, R, k_idxs, grads, gnames):
 
 
 def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, k_idxs, *,
-                                 key: str = "aniso", csr=None, needs=None, cache=None):
+                                 key: str = "aniso", csr=None, needs=None, cache=None, check: bool = True):
     """First-order gradients -> ``[dq, dp, df, dsigma, dR]``.
 
     Reference: ``knn_aggregate_aniso_backward`` (csrc/knn_aggregate.cpp:101-132,
@@ -496,12 +597,21 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     (csrc/backward.cu:93-94); ``w_out`` is not read either -- weights are recomputed
     from ``k_idxs``.  ``needs`` (5 bools) skips outputs (returned as ``None``);
     ``csr`` = :func:`build_csr` result, built here when point gradients are needed.
+    ``grad_f_out`` may be a broadcast view (:func:`broadcast_rows`).  ``k_idxs`` is
+    range-checked unless ``check=False`` (the kernels trust it; the autograd Functions
+    pass indices from this library's own forward).
     """
-    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, (grad_f_out,), ("grad_f_out",))
-    if tuple(grad_f_out.shape) != (B, Q, F):
+    rows = _bcast_of(grad_f_out, q, f)
+    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, (grad_f_out if rows is None else rows,),
+                                    ("grad_f_out",))
+    if rows is None and tuple(grad_f_out.shape) != (B, Q, F):
         raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
     names = ("dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    if B * Q == 0:
+        return _no_queries(names, mask, B, Q, P, F, dt, q.device)
+    if check:
+        _check_range(k_idxs, P, "k_idxs")
     slabs = _feature_slabs(F, dt)
     if slabs is not None:
         if mask & _lib.NEED_POINTS and csr is None:
@@ -509,8 +619,9 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
         df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
         res = None
         for s0, s1 in slabs:
-            o = knn_aggregate_aniso_backward(_cols(grad_f_out, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R,
-                                             w_out, k_idxs, key=key, csr=csr, needs=needs, cache=cache)
+            o = knn_aggregate_aniso_backward(_gcols(grad_f_out, rows, s0, s1), grad_w_out, q, p, _cols(f, s0, s1),
+                                             sigma, R, w_out, k_idxs, key=key, csr=csr, needs=needs, cache=cache,
+                                             check=False)
             if df is not None:
                 df[..., s0:s1] = o[2]
             if res is None:
@@ -535,6 +646,9 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     dims = make_dims(B, Q, P, F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
+    if rows is not None:
+        mask |= _lib.GRAD_BROADCAST
+        grad_f_out = rows
     ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     with torch.cuda.device(dev):
@@ -550,7 +664,7 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
 def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out,
                                      k_idxs, *, key: str = "aniso", csr=None, needs=None, cache=None,
                                      grad_grad_p=None, grad_grad_f=None, grad_grad_sigma=None,
-                                     grad_grad_R=None):
+                                     grad_grad_R=None, check: bool = True):
     """Second-order gradients -> ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
 
     Reference: ``knn_aggregate_aniso_backward_2nd`` (csrc/knn_aggregate.cpp:135-172,
@@ -558,17 +672,22 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
     upstreams ``grad_grad_p`` [B,P,3], ``grad_grad_f`` [B,P,F], ``grad_grad_sigma`` [B,P,3]
     and ``grad_grad_R`` [B,P,3,3] add the terms the reference drops
     (cuda_knn_aggregate/__init__.py:271-289); ``grad_grad_q`` may then be ``None``.
-    ``needs`` (6 bools) skips outputs (``None``).
+    ``needs`` (6 bools) skips outputs (``None``).  ``grad_f_out`` and ``check`` as in
+    :func:`knn_aggregate_aniso_backward`.
     """
     ups = {"grad_grad_p": (grad_grad_p, 3), "grad_grad_f": (grad_grad_f, None),
            "grad_grad_sigma": (grad_grad_sigma, 3), "grad_grad_R": (grad_grad_R, (3, 3))}
     full = any(t is not None for t, _ in ups.values())
     if grad_grad_q is None and not full:
         raise RuntimeError("grad_grad_q is required (or one of grad_grad_p / _f / _sigma / _R)")
+    rows = None if full else _bcast_of(grad_f_out, q, f)
+    if full and isinstance(grad_f_out, torch.Tensor) and _bcast_of(grad_f_out, q, f) is not None:
+        grad_f_out = grad_f_out.contiguous()  # the full second order takes a materialised upstream
     given = [(t, n) for n, (t, _) in [("grad_grad_q", (grad_grad_q, 0))] + list(ups.items()) if t is not None]
-    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs, tuple(t for t, _ in given) + (grad_f_out,),
+    dt, B, Q, P, F, K = _bwd_common(q, p, f, sigma, R, k_idxs,
+                                    tuple(t for t, _ in given) + (grad_f_out if rows is None else rows,),
                                     tuple(n for _, n in given) + ("grad_f_out",))
-    if tuple(grad_f_out.shape) != (B, Q, F):
+    if rows is None and tuple(grad_f_out.shape) != (B, Q, F):
         raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
     if grad_grad_q is not None and tuple(grad_grad_q.shape) != (B, Q, 3):
         raise RuntimeError(f"grad_grad_q must have shape [B, Q, 3] (got {tuple(grad_grad_q.shape)})")
@@ -578,6 +697,10 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
             raise RuntimeError(f"{n} must have shape {list(want)} (got {tuple(t.shape)})")
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    if B * Q == 0:
+        return _no_queries(names, mask, B, Q, P, F, dt, q.device)
+    if check:
+        _check_range(k_idxs, P, "k_idxs")
     slabs = _feature_slabs(F, dt)
     if slabs is not None:
         if (mask & _lib.NEED_POINTS or (full and mask & _lib.NEED_DQ)) and csr is None:
@@ -587,9 +710,11 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         res = None
         for s0, s1 in slabs:
             o = knn_aggregate_aniso_backward_2nd(
-                grad_grad_q, _cols(grad_f_out, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R, w_out, k_idxs,
+                grad_grad_q, _gcols(grad_f_out, rows, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R, w_out,
+                k_idxs,
                 key=key, csr=csr, needs=needs, cache=cache, grad_grad_p=grad_grad_p,
-                grad_grad_f=_cols(grad_grad_f, s0, s1), grad_grad_sigma=grad_grad_sigma, grad_grad_R=grad_grad_R)
+                grad_grad_f=_cols(grad_grad_f, s0, s1), grad_grad_sigma=grad_grad_sigma, grad_grad_R=grad_grad_R,
+                check=False)
             if dgf is not None:
                 dgf[..., s0:s1] = o[0]
             if df is not None:
@@ -617,6 +742,9 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
     dims = make_dims(B, Q, P, F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
+    if rows is not None:
+        mask |= _lib.GRAD_BROADCAST
+        grad_f_out = rows
     ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask | (_lib.SECOND_FULL if full else 0))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     if full:

@@ -104,8 +104,10 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
     def forward(ctx, q, p, f, sigma, R, K, key="aniso"):
         # The cache (spatial row orders, packed point records) only speeds up the
         # backward kernels; it is filled when a backward can follow.
+        # (none without query rows: the forward then writes nothing into it)
         cache = None
-        if any(ctx.needs_input_grad[:5]) and q.is_cuda and p.dim() == 3 and q.dim() == 3:
+        if (any(ctx.needs_input_grad[:5]) and q.is_cuda and p.dim() == 3 and q.dim() == 3
+                and q.shape[0] * q.shape[1] > 0):
             cache = _C.new_cache(q.shape[0], q.shape[1], p.shape[1], q.dtype, q.device)
         f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key, cache=cache)
         # Like the reference (__init__.py:89); the backward kernels recompute the
@@ -129,8 +131,10 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         needs = tuple(ctx.needs_input_grad[:5])
         # w_out is detached: the backward never reads it, and an edge from the
         # second-order graph back into this node would only run it again on zeros.
+        # A broadcast upstream (v.expand_as(f_out), the Eikonal read-out) stays a view: the
+        # library reads its rows instead of a materialised [B, Q, F] copy (same results).
         grad_q, grad_p, grad_f, grad_sigma, grad_R = KNNAnisotropicAggregateBackward.apply(
-            grad_f_out.contiguous(),
+            _C.upstream(grad_f_out),
             grad_w_out,  # never read
             q,
             p,
@@ -199,10 +203,11 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         if needs is None:
             needs = (True,) * 5
         P = p.shape[1]
-        csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) else None
+        has_q = q.shape[0] * q.shape[1] > 0
+        csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) and has_q else None
         dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward(
             grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs,
-            cache=cache)
+            cache=cache, check=False)
         saved = [grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs]
         if csr is not None:
             saved += list(csr)
@@ -232,11 +237,11 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         full = any(u is not None for u in ups)
         if (grad_grad_q is None and not full) or not any(needs):
             return (None,) * 12
-        if csr is None and (_point_needs(needs[2:]) or (full and needs[1])):
+        if csr is None and (_point_needs(needs[2:]) or (full and needs[1])) and q.shape[0] * q.shape[1] > 0:
             csr = _C.build_csr(idxs, p.shape[1], ctx.key)
         d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
             None if grad_grad_q is None else grad_grad_q.contiguous(),
-            grad_f_out.contiguous(),
+            _C.upstream(grad_f_out),
             grad_w_out,  # never read
             q,
             p,
@@ -253,5 +258,6 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
             grad_grad_f=ups[1],
             grad_grad_sigma=ups[2],
             grad_grad_R=ups[3],
+            check=False,
         )
         return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None, None

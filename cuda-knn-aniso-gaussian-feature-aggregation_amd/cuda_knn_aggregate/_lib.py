@@ -16,7 +16,7 @@ import torch  # noqa: F401  (loads the HIP runtime the library binds to)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libaknn.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 AKNN_OK = 0
 AKNN_E_INVALID = -1
@@ -35,6 +35,7 @@ NEED_DSIGMA = 8
 NEED_DR = 16
 NEED_DGRAD_F_OUT = 32
 SECOND_FULL = 64
+GRAD_BROADCAST = 128  # grad_f_out given as one row per cloud ([B, F])
 NEED_POINTS = NEED_DP | NEED_DF | NEED_DSIGMA | NEED_DR
 
 EXPORTS = (
@@ -59,6 +60,7 @@ EXPORTS = (
     "aknn_lookup_forward",
     "aknn_iso_workspace_size",
     "aknn_iso_forward",
+    "aknn_point_dots",
 )
 
 
@@ -84,7 +86,7 @@ class Cache(ctypes.Structure):
     """``aknn_cache`` (include/aknn.h)."""
 
     _fields_ = [("query_order", ctypes.c_void_p), ("point_order", ctypes.c_void_p),
-                ("point_records", ctypes.c_void_p)]
+                ("point_records", ctypes.c_void_p), ("point_dots", ctypes.c_void_p)]
 
 
 class Tune(ctypes.Structure):
@@ -131,6 +133,7 @@ _SIGS = {
                             + [_P, ctypes.c_size_t, _P]),
     "aknn_iso_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_iso_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 4 + [_P] * 4 + [_P, ctypes.c_size_t, _P]),
+    "aknn_point_dots": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P]),
 }
 
 

@@ -129,21 +129,48 @@ def build_csr(layout, idxs, K):
     return off, ent
 
 
-def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, needs, cache):
+def _rows(layout, g):
+    """[B, F] rows of a ragged upstream [Q, F] that repeats one row for every query (a
+    stride-0 view such as ``v.expand(Q, F)``), or None."""
+    if g.dim() != 2 or g.shape[0] < 1 or g.stride(0) != 0 or (g.stride(1) != 1 and g.shape[1] > 1):
+        return None
+    return g[:1].expand(layout.B, g.shape[1]).contiguous()
+
+
+def _upstream(g):
+    return g if g.dim() == 2 and g.shape[0] >= 1 and g.stride(0) == 0 and (g.stride(1) == 1 or g.shape[1] == 1) \
+        else g.contiguous()
+
+
+def _gcols(g, s0, s1):
+    return g[:, s0:s1] if g.stride(0) == 0 else _C._cols(g, s0, s1)
+
+
+def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, needs, cache, ups=None):
     dt, F = _check(layout, q, p, f, sigma, R)
     K = int(idxs.shape[1])
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR") if second else ("dq", "dp", "df", "dsigma", "dR")
     mask = _C._needs_mask(needs, names)
+    full = ups is not None and any(u is not None for u in ups)
+    if full and g.stride(0) == 0:
+        g = g.contiguous()  # the full second order takes a materialised upstream
+    dev = q.device
+    Q, P = layout.Q, layout.P
+    shapes = {"dgf": (Q, F), "dq": (Q, 3), "dp": (P, 3), "df": (P, F), "dsigma": (P, 3), "dR": (P, 3, 3)}
+    if Q == 0:  # no query rows: nothing reaches the points
+        return [torch.zeros(shapes[n], dtype=dt, device=dev) if mask & _C._NEED_BITS[n] else None for n in names]
+    need_csr = bool(mask & _lib.NEED_POINTS) or (full and bool(mask & _lib.NEED_DQ))
     slabs = _C._feature_slabs(F, dt)
     if slabs is not None:  # column ranges: column outputs side by side, the rest summed in order
-        if mask & _lib.NEED_POINTS and csr is None:
+        if need_csr and csr is None:
             csr = build_csr(layout, idxs, K)
         cols = {n: (torch.empty(layout.Q if n == "dgf" else layout.P, F, dtype=dt, device=q.device)
                     if mask & _C._NEED_BITS[n] else None) for n in ("dgf", "df") if n in names}
         res = None
         for s0, s1 in slabs:
-            o = _backward_impl(layout, second, gg, _C._cols(g, s0, s1), q, p, _C._cols(f, s0, s1), sigma, R, idxs,
-                               key, csr, needs, cache)
+            ups_c = None if ups is None else (ups[0], _C._cols(ups[1], s0, s1), ups[2], ups[3])
+            o = _backward_impl(layout, second, gg, _gcols(g, s0, s1), q, p, _C._cols(f, s0, s1), sigma, R, idxs,
+                               key, csr, needs, cache, ups_c)
             for n, t in cols.items():
                 if t is not None:
                     t[:, s0:s1] = o[names.index(n)]
@@ -156,21 +183,28 @@ def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, nee
         for n, t in cols.items():
             res[names.index(n)] = t
         return res
-    dev = q.device
-    Q, P = layout.Q, layout.P
-    shapes = {"dgf": (Q, F), "dq": (Q, 3), "dp": (P, 3), "df": (P, F), "dsigma": (P, 3), "dR": (P, 3, 3)}
     outs = {n: (torch.empty(shapes[n], dtype=dt, device=dev) if mask & _C._NEED_BITS[n] else None) for n in names}
-    if mask & _lib.NEED_POINTS and csr is None:
+    if need_csr and csr is None:
         csr = build_csr(layout, idxs, K)
     lib = _lib.load()
     dims = layout.dims(F, K, dt, key)
     off, ent = csr if csr is not None else (None, None)
     P_ = _lib.ptr
-    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
+    rows = None if full else _rows(layout, g)
+    if rows is not None:
+        mask |= _lib.GRAD_BROADCAST
+        g = rows
+    ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask | (_lib.SECOND_FULL if full else 0))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     cs = _C._cp(_C._cache_struct(cache))
     with torch.cuda.device(dev):
-        if second:
+        if full:
+            rc = lib.aknn_backward2_full(
+                ctypes.byref(dims), mask, P_(gg), P_(ups[0]), P_(ups[1]), P_(ups[2]), P_(ups[3]), g.data_ptr(),
+                q.data_ptr(), p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(), idxs.data_ptr(), P_(off),
+                P_(ent), cs, P_(ws), ws_bytes, P_(outs["dgf"]), P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]),
+                P_(outs["dsigma"]), P_(outs["dR"]), _C._stream(dev))
+        elif second:
             rc = lib.aknn_backward2(ctypes.byref(dims), mask, gg.data_ptr(), g.data_ptr(), q.data_ptr(),
                                     p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(), idxs.data_ptr(),
                                     P_(off), P_(ent), cs, P_(ws), ws_bytes, P_(outs["dgf"]), P_(outs["dq"]),
@@ -186,14 +220,21 @@ def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, nee
 
 
 def backward(layout, g, q, p, f, sigma, R, idxs, *, key="aniso", csr=None, needs=None, cache=None):
-    """First-order gradients -> ``[dq, dp, df, dsigma, dR]``."""
-    return _backward_impl(layout, False, None, g.contiguous(), q, p, f, sigma, R, idxs, key, csr, needs, cache)
+    """First-order gradients -> ``[dq, dp, df, dsigma, dR]``.  ``g`` [Q, F], or a view
+    repeating one row for every query (read as such, same results)."""
+    return _backward_impl(layout, False, None, _upstream(g), q, p, f, sigma, R, idxs, key, csr, needs, cache)
 
 
-def backward2(layout, gg, g, q, p, f, sigma, R, idxs, *, key="aniso", csr=None, needs=None, cache=None):
-    """Second-order gradients of ``<gg, dq>`` -> ``[d grad_f_out, dq, dp, df, dsigma, dR]``."""
-    return _backward_impl(layout, True, gg.contiguous(), g.contiguous(), q, p, f, sigma, R, idxs, key, csr,
-                          needs, cache)
+def backward2(layout, gg, g, q, p, f, sigma, R, idxs, *, key="aniso", csr=None, needs=None, cache=None,
+              gg_p=None, gg_f=None, gg_sigma=None, gg_R=None):
+    """Second-order gradients -> ``[d grad_f_out, dq, dp, df, dsigma, dR]`` of
+    ``<gg, dq> + <gg_p, dp> + <gg_f, df> + <gg_sigma, dsigma> + <gg_R, dR>`` (each upstream
+    optional; ``gg`` alone is the reference's second order)."""
+    ups = tuple(None if u is None else u.contiguous() for u in (gg_p, gg_f, gg_sigma, gg_R))
+    if gg is None and all(u is None for u in ups):
+        raise RuntimeError("backward2 needs gg or one of gg_p / gg_f / gg_sigma / gg_R")
+    return _backward_impl(layout, True, None if gg is None else gg.contiguous(), _upstream(g), q, p, f, sigma, R,
+                          idxs, key, csr, needs, cache, ups)
 
 
 class KNNAnisotropicAggregateRagged(torch.autograd.Function):
@@ -201,7 +242,8 @@ class KNNAnisotropicAggregateRagged(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, q, p, f, sigma, R, layout, K, key="aniso", tune=None):
-        cache = layout.new_cache(q.dtype) if any(ctx.needs_input_grad[:5]) else None
+        # (none without query rows: the forward then writes nothing into it)
+        cache = layout.new_cache(q.dtype) if any(ctx.needs_input_grad[:5]) and layout.Q > 0 else None
         f_out, w_out, dists, idxs = forward(layout, q, p, f, sigma, R, K, key=key, tune=tune, cache=cache)
         ctx.save_for_backward(q, p, f, sigma, R, idxs, *(cache or ()))
         ctx.layout, ctx.key, ctx.has_cache = layout, key, cache is not None
@@ -216,18 +258,20 @@ class KNNAnisotropicAggregateRagged(torch.autograd.Function):
         if grad_f_out is None:
             return (None,) * 9
         needs = tuple(ctx.needs_input_grad[:5])
-        grads = KNNAnisotropicAggregateRaggedBackward.apply(grad_f_out.contiguous(), q, p, f, sigma, R, idxs,
+        grads = KNNAnisotropicAggregateRaggedBackward.apply(_upstream(grad_f_out), q, p, f, sigma, R, idxs,
                                                             ctx.layout, ctx.key, needs, cache)
         return tuple(grads) + (None,) * 4
 
 
 class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
-    """First order as a differentiable op; its backward is the Eikonal second order
-    (only the gradient arriving on dq is honoured, as in the dense Function)."""
+    """First order as a differentiable op; its backward is the second order.  As in the
+    dense Function, upstream gradients on dp, df, dsigma and dR enter too unless
+    ``cuda_knn_aggregate.FULL_SECOND_ORDER`` is False (then only the one on dq, as in the
+    reference)."""
 
     @staticmethod
     def forward(ctx, grad_f_out, q, p, f, sigma, R, idxs, layout, key, needs, cache):
-        csr = build_csr(layout, idxs, int(idxs.shape[1])) if any(needs[1:]) else None
+        csr = build_csr(layout, idxs, int(idxs.shape[1])) if any(needs[1:]) and layout.Q > 0 else None
         out = backward(layout, grad_f_out, q, p, f, sigma, R, idxs, key=key, csr=csr, needs=needs, cache=cache)
         saved = [grad_f_out, q, p, f, sigma, R, idxs] + (list(csr) if csr else []) + (list(cache) if cache else [])
         ctx.save_for_backward(*saved)
@@ -243,12 +287,16 @@ class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
         cache = tuple(rest[:3]) if ctx.has_cache else None
         ni = ctx.needs_input_grad
         needs = (ni[0], ni[1], ni[2], ni[3], ni[4], ni[5])
-        if gg_q is None or not any(needs):
+        from . import FULL_SECOND_ORDER  # noqa: PLC0415  (the package's switch, read per call)
+
+        ups = (gg_p, gg_f, gg_sigma, gg_R) if FULL_SECOND_ORDER else (None,) * 4
+        full = any(u is not None for u in ups)
+        if (gg_q is None and not full) or not any(needs):
             return (None,) * 11
-        if csr is None and any(needs[2:]):
+        if csr is None and (any(needs[2:]) or (full and needs[1])) and ctx.layout.Q > 0:
             csr = build_csr(ctx.layout, idxs, int(idxs.shape[1]))
         out = backward2(ctx.layout, gg_q, grad_f_out, q, p, f, sigma, R, idxs, key=ctx.key, csr=csr,
-                        needs=needs, cache=cache)
+                        needs=needs, cache=cache, gg_p=ups[0], gg_f=ups[1], gg_sigma=ups[2], gg_R=ups[3])
         return tuple(out) + (None,) * 5
 
 
@@ -258,6 +306,7 @@ def knn_aggregate_aniso_ragged(q, p, f, sigma, R, layout: RaggedLayout, K: int, 
 
     Returns ``f_out [sum Q_i, F]``, ``w_out``, ``dists``, ``idxs [sum Q_i, K]`` (indices
     within each query's own cloud); differentiable in q, p, f, sigma, R with the same first-
-    and second-order semantics as the dense call.
+    and second-order semantics as the dense call (including upstream gradients on the
+    first-order dp, df, dsigma and dR under ``FULL_SECOND_ORDER``).
     """
     return KNNAnisotropicAggregateRagged.apply(q, p, f, sigma, R, layout, K, key, tune)

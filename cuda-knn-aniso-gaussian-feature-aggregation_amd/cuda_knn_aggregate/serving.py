@@ -48,6 +48,7 @@ class Request:
     normal: Optional[torch.Tensor] = None
     rid: int = 0
     row: int = 0                     # first row of its queries in the batch
+    ready: Optional[torch.cuda.Event] = None  # device queries: recorded on their producer's stream
 
     @property
     def latency(self) -> float:
@@ -123,8 +124,13 @@ class Server:
                                     offsets=(q0.to(self.device), self.p_off, q0, self.p_off_host))
         self.tune = _C._tune(tune) or _lib.Tune()
         self.tune.index = self.index.data_ptr()
-        # the read-out row v for every query slot of the largest batch, built once
-        self.g_all = self.v.expand(self.max_batch_queries, self.F).contiguous()
+        # The read-out v is a broadcast upstream (one row per cloud): its point dot
+        # products a[n] = v . f[n] are fixed for the resident clouds, so they are formed
+        # once here and every batch's dq reads them (no feature rows gathered per batch).
+        self.g_rows = self.v.expand(self.n_clouds, self.F).contiguous()
+        self.dots = _C.point_dots(self.g_rows, self.f, self.p, self.sigma, self.R,
+                                  offsets=(q0.to(self.device), self.p_off, q0, self.p_off_host))
+        self.dq_cache = _C._cache_struct((None, None, None, self.dots))
         ws = self._ws_bytes(self.max_batch_queries)
         self.slots = [_Slot(self.device, self.dtype, self.F, self.K, self.max_batch_queries, self.n_clouds, ws)
                       for _ in range(streams)]
@@ -136,6 +142,9 @@ class Server:
         self.t0 = time.perf_counter()
         self.batches = 0
         self.on_done = on_done
+        # The set-up above ran on the current stream; the slot streams do not wait for it
+        # by themselves, so finish it before any batch can be launched.
+        torch.cuda.current_stream(self.device).synchronize()
 
     def _dims(self, nq, q_off, q_off_host):
         return _C.make_dims(self.n_clouds, nq, self.P, self.F, self.K, self.dtype, self.key,
@@ -173,6 +182,9 @@ class Server:
             raise ValueError(f"request of {req.q.shape[0]} queries exceeds max_batch_queries")
         if self.batch_bytes(req.q.shape[0]) > self.hbm_budget:
             raise ValueError("request alone exceeds the HBM budget")
+        if req.q.is_cuda:  # device queries: the slot stream must see their producer's work
+            req.ready = torch.cuda.Event()
+            req.ready.record(torch.cuda.current_stream(req.q.device))
         self.waiting.append(req)
 
     def batch_bytes(self, nq: int) -> int:
@@ -200,6 +212,8 @@ class Server:
         with torch.cuda.stream(st):
             sl.q_off.copy_(sl.q_off_host, non_blocking=True)
             for r in reqs:
+                if r.ready is not None:
+                    st.wait_event(r.ready)
                 m = r.q.shape[0]
                 r.started = t
                 r.row = fill[r.cloud]
@@ -218,10 +232,10 @@ class Server:
                                     self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(), self.R.data_ptr(),
                                     sl.f_out.data_ptr(), sl.w_out.data_ptr(), sl.dists.data_ptr(),
                                     sl.idxs.data_ptr(), sl.ws.data_ptr(), ws_bytes, None, hs))
-        _lib.check(lib.aknn_backward(ctypes.byref(dims), _lib.NEED_DQ, self.g_all.data_ptr(), sl.q.data_ptr(),
-                                     self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(),
-                                     self.R.data_ptr(), sl.idxs.data_ptr(), None, None, None, None, 0,
-                                     sl.dq.data_ptr(), None, None, None, None, hs))
+        _lib.check(lib.aknn_backward(ctypes.byref(dims), _lib.NEED_DQ | _lib.GRAD_BROADCAST, self.g_rows.data_ptr(),
+                                     sl.q.data_ptr(), self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(),
+                                     self.R.data_ptr(), sl.idxs.data_ptr(), None, None, ctypes.byref(self.dq_cache),
+                                     None, 0, sl.dq.data_ptr(), None, None, None, None, hs))
         with torch.cuda.stream(st):
             sl.f_host[:nq].copy_(sl.f_out[:nq], non_blocking=True)
             sl.n_host[:nq].copy_(sl.dq[:nq], non_blocking=True)

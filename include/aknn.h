@@ -46,7 +46,7 @@ extern "C" {
 #define AKNN_API
 #endif
 
-#define AKNN_ABI_VERSION 1
+#define AKNN_ABI_VERSION 2
 
 /* error codes */
 #define AKNN_OK 0
@@ -75,6 +75,17 @@ extern "C" {
 #define AKNN_NEED_DGRAD_F_OUT 32u /* aknn_backward2 only */
 #define AKNN_SECOND_FULL 64u      /* workspace query for aknn_backward2_full (see there) */
 #define AKNN_NEED_POINTS (AKNN_NEED_DP | AKNN_NEED_DF | AKNN_NEED_DSIGMA | AKNN_NEED_DR)
+/* Modifier bit for aknn_backward / aknn_backward2 (not aknn_backward2_full): grad_f_out
+ * is a broadcast upstream, one row of F values per cloud shared by all of that cloud's
+ * query rows, passed as a [B, F] array (the reference materialises such an upstream,
+ * e.g. the Eikonal read-out v.expand_as(f_out), with .contiguous() at
+ * cuda_knn_aggregate/__init__.py:107).  Results equal those for the materialised
+ * [B, Q, F] upstream: the point gradients bit for bit; dq to rounding (it is then summed
+ * per query from point dot products a[n] = grad_f_out[cloud(n)] . f[n], see
+ * aknn_point_dots, with the bits of the per-entry shares a materialised upstream gets
+ * when point gradients are requested too).  The point kernels then form a[n] once per
+ * point row instead of gathering one upstream row per neighbour entry. */
+#define AKNN_GRAD_BROADCAST 128u
 
 /* Problem description.  Dense batches: q_off = p_off = NULL and B, Q, P give the
  * shape.  Ragged batches: q_off / p_off are DEVICE arrays of B+1 int64 row
@@ -138,6 +149,11 @@ typedef struct aknn_cache {
   int32_t* query_order;  /* query rows in spatial order */
   int32_t* point_order;  /* point rows in spatial order */
   void* point_records;
+  /* dtype[point rows][16]: the dot records of aknn_point_dots, for backward calls with
+   * AKNN_GRAD_BROADCAST and the same grad_f_out rows and points (a caller that keeps
+   * both fixed, like a server with a fixed read-out, makes them once); NULL = made per
+   * call in the workspace.  Not filled by aknn_forward. */
+  const void* point_dots;
 } aknn_cache;
 
 AKNN_API int aknn_abi_version(void);
@@ -235,14 +251,25 @@ AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t*
 
 /* Device workspace of aknn_backward / aknn_backward2 for a `needs` mask (0 when none is
  * needed): with dq and point gradients both requested, the point kernels hand each
- * (query, slot) entry's dq share to a per-query sum instead of a second gather. */
+ * (query, slot) entry's dq share to a per-query sum instead of a second gather.  With
+ * AKNN_GRAD_BROADCAST and dq requested: the dot records (dtype[point rows][16]),
+ * unless the cache brings them. */
 AKNN_API size_t aknn_backward_workspace_size(const aknn_dims* dims, uint32_t needs);
+
+/* Dot records for a broadcast upstream (no counterpart in the reference): for every
+ * point row n, 16 values {p0 p1 p2, R00..R22, s0 s1 s2, a} with
+ * a = grad_rows[cloud(n)] . f[n] in the association the backward kernels use (same
+ * bits); grad_rows [B, F], out dtype[point rows][16].  dq of a broadcast upstream reads
+ * a neighbour's geometry and dot from one record.  For aknn_cache.point_dots. */
+AKNN_API int aknn_point_dots(const aknn_dims* dims, const void* grad_rows, const void* f, const void* p,
+                             const void* sigma, const void* R, void* out, void* stream);
 
 /* First-order backward.  Replaces knn_aggregate_aniso_backward
  * (csrc/knn_aggregate.cpp:101-132).  Gaussian weights are recomputed from idxs;
  * grad_w_out is not an input (the reference ignores it, csrc/backward.cu:93-94).
  * Outputs selected by `needs` are fully overwritten; others may be NULL.  The
- * point outputs need the CSR from aknn_build_csr. */
+ * point outputs need the CSR from aknn_build_csr.  grad_f_out is [B, Q, F], or [B, F]
+ * with AKNN_GRAD_BROADCAST in needs. */
 AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
@@ -251,7 +278,8 @@ AKNN_API int aknn_backward(const aknn_dims* dims, uint32_t needs, const void* gr
 
 /* Second-order backward of <grad_grad_q, dq>.  Replaces knn_aggregate_aniso_backward_2nd
  * (csrc/knn_aggregate.cpp:135-172); like the reference only grad_grad_q is an input
- * (cuda_knn_aggregate/__init__.py:260-289). */
+ * (cuda_knn_aggregate/__init__.py:260-289).  grad_f_out as in aknn_backward (the
+ * broadcast bit applies here too; d_grad_f_out stays [B, Q, F]). */
 AKNN_API int aknn_backward2(const aknn_dims* dims, uint32_t needs, const void* grad_grad_q,
                    const void* grad_f_out, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs,

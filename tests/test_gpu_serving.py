@@ -45,7 +45,9 @@ def test_results_same_alone_and_in_the_mix(cuda):
         p, f, s, R = (t.to(cuda)[None].contiguous() for t in clouds[r.cloud])
         q = r.q.to(cuda)[None]
         f_out, w_out, _, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, server.K)
-        g = server.v.expand(1, q.shape[1], server.F).contiguous()
+        # the read-out as a broadcast upstream, as the server takes it (dq summed from the
+        # point dot products v . f[n]: the same bits in a dense call on the cloud alone)
+        g = server.v.expand(1, q.shape[1], server.F)
         dq = _C.knn_aggregate_aniso_backward(g, w_out, q, p, f, s, R, w_out, idxs,
                                              needs=(True, False, False, False, False))[0]
         assert torch.equal(r.f_out, f_out[0].cpu()) and torch.equal(r.normal, dq[0].cpu())
