@@ -35,7 +35,9 @@ def make_requests(clouds, n, seed=0, rate=2000.0, log_q=(5.5, 8.5)):
 
 def test_results_same_alone_and_in_the_mix(cuda):
     server, clouds = make_server(cuda, streams=3, max_batch_queries=6000)
-    reqs = make_requests(clouds, 40)
+    # arrivals within a few microseconds of each other: requests must share batches
+    # (at a realistic rate the batches may finish before the next request arrives)
+    reqs = make_requests(clouds, 40, rate=1e6)
     server.replay(reqs)
     assert all(r.done >= r.arrival for r in reqs)
     assert server.batches < len(reqs)          # some requests shared a batch
