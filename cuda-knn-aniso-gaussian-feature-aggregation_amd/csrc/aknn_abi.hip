@@ -415,18 +415,13 @@ int aknn_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const voi
   return run_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, static_cast<hipStream_t>(stream));
 }
 
-int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
-                 const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
-                 void* dists, int64_t* idxs, void* ws, size_t ws_bytes, aknn_cache* cache,
-                 void* stream) {
-  int rc = check_dims(d, true);
-  if (rc) return rc;
-  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
-  if (!q || !p || !sigma || !R || !f || !f_out || !w_out || !dists || !idxs)
-    return fail(AKNN_E_INVALID, "null tensor pointer");
-  if (!ws || ws_bytes < aknn_forward_workspace_size(d))
-    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
-                aknn_forward_workspace_size(d));
+}  // extern "C"
+
+// Search + aggregation into a workspace already checked to suffice for this tune.
+static int forward_run(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p, const void* f,
+                       const void* sigma, const void* R, void* f_out, void* w_out, void* dists, int64_t* idxs,
+                       void* ws, size_t ws_bytes, aknn_cache* cache, void* stream) {
+  int rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const uint32_t *qord = nullptr, *pord = nullptr;
   rc = run_knn(d, tune, q, p, sigma, R, dists, idxs, ws, ws_bytes, st, &qord, &pord);
@@ -476,6 +471,108 @@ int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const
   }
   if (e != hipSuccess) return hip_fail(e, "aggregate");
   return AKNN_OK;
+}
+
+extern "C" {
+
+int aknn_forward(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p,
+                 const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
+                 void* dists, int64_t* idxs, void* ws, size_t ws_bytes, aknn_cache* cache,
+                 void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!q || !p || !sigma || !R || !f || !f_out || !w_out || !dists || !idxs)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  if (!ws || ws_bytes < aknn_forward_workspace_size(d))
+    return fail(AKNN_E_WORKSPACE, "workspace too small (%zu < %zu bytes)", ws_bytes,
+                aknn_forward_workspace_size(d));
+  return forward_run(d, tune, q, p, f, sigma, R, f_out, w_out, dists, idxs, ws, ws_bytes, cache, stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free forward
+// Device memory the allocation-free forward uses in place of a caller's workspace: one
+// copy per GPU, in the library's code object, held until the process ends.  Host code
+// never passes `scratch` itself (in host code it is only a stand-in for the symbol):
+// each device's address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = size_t(AKNN_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch` (looked up on the device's first call).
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "forward_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(AKNN_E_UNSUPPORTED, "forward_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                 "forward_noalloc: hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return AKNN_OK;
+}
+
+// Workspace bytes the forward needs with the tiled search split `ways` ways (or the brute
+// scan's, for the shapes that take it).
+size_t forward_need(const aknn_dims* d, const aknn_tune* tune, int ways) {
+  const size_t brute = rec_bytes(d) + align256(static_cast<size_t>(d->B + 1) * 8);
+  if (pick_search(d, tune) != AKNN_SEARCH_TILES) return brute;
+  aknn::PrunedLaunch PL = pruned_launch(d);
+  PL.ways = ways;
+  PL.ways_max = ways;
+  PL.index = tune ? tune->index : nullptr;  // a prebuilt point index is not carved from the workspace
+  size_t bytes = 0;
+  if (aknn::pruned_workspace(PL, &bytes) != hipSuccess) return SIZE_MAX;
+  return bytes;
+}
+}  // namespace
+
+extern "C" {
+
+int aknn_forward_noalloc(const aknn_dims* d, const aknn_tune* tune, const void* q, const void* p, const void* f,
+                         const void* sigma, const void* R, void* f_out, void* w_out, void* dists, int64_t* idxs,
+                         aknn_cache* cache, void* stream) {
+  int rc = check_dims(d, true);
+  if (rc) return rc;
+  if (query_rows(d) == 0 || d->B == 0) return AKNN_OK;
+  if (!q || !p || !sigma || !R || !f || !f_out || !w_out || !dists || !idxs)
+    return fail(AKNN_E_INVALID, "null tensor pointer");
+  // The tiled search's split walk keeps per-way lists; where the library's split for
+  // the shape does not fit the scratch, take the largest split that does (every split
+  // gives the same results).
+  aknn_tune t = tune ? *tune : aknn_tune{};
+  int ways = 1;
+  if (pick_search(d, tune) == AKNN_SEARCH_TILES) {
+    const aknn::PrunedLaunch PL = pruned_launch(d);
+    ways = PL.ways;
+    if (tune && tune->point_split > 0) {
+      int w = 1;
+      while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;
+      ways = w;
+    }
+    while (ways > 1 && forward_need(d, tune, ways) > kScratchBytes) ways /= 2;
+    t.point_split = ways;
+  }
+  const size_t need = forward_need(d, tune, ways);
+  if (need > kScratchBytes)
+    return fail(AKNN_E_UNSUPPORTED,
+                "forward_noalloc: this batch needs %zu bytes of scratch, more than the %zu the allocation-free "
+                "entry holds (use aknn_forward with a workspace)", need, kScratchBytes);
+  void* ws = nullptr;
+  rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  return forward_run(d, &t, q, p, f, sigma, R, f_out, w_out, dists, idxs, ws, kScratchBytes, cache, stream);
 }
 
 int aknn_aggregate(const aknn_dims* d, const void* q, const void* p, const void* f,

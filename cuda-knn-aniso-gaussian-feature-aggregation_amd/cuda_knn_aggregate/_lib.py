@@ -61,6 +61,7 @@ EXPORTS = (
     "aknn_iso_workspace_size",
     "aknn_iso_forward",
     "aknn_point_dots",
+    "aknn_forward_noalloc",
 )
 
 
@@ -134,7 +135,9 @@ _SIGS = {
     "aknn_iso_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_iso_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 4 + [_P] * 4 + [_P, ctypes.c_size_t, _P]),
     "aknn_point_dots": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P]),
+    "aknn_forward_noalloc": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 9 + [_CACHE, _P]),
 }
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 def load(path: str | None = None):

@@ -183,6 +183,22 @@ AKNN_API int aknn_forward(const aknn_dims* dims, const aknn_tune* tune, const vo
                  void* dists, int64_t* idxs, void* workspace, size_t workspace_bytes,
                  aknn_cache* cache, void* stream);
 
+/* Allocation-free forward (no counterpart in the reference: for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same as aknn_forward, with the same results, but without a
+ * workspace argument: the library's own per-device scratch array of
+ * AKNN_NOALLOC_SCRATCH_BYTES (in its code object, held until the process ends) takes the
+ * workspace's place, and no call allocates, frees or runs an allocating routine.  Where
+ * the tiled search's default split walk for the shape would not fit, a smaller split
+ * (same results) is taken; a batch that needs more scratch than that even unsplit fails
+ * with AKNN_E_UNSUPPORTED and the sizes in the message.  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (different streams, or threads)
+ * race and must be serialised by the caller. */
+#define AKNN_NOALLOC_SCRATCH_BYTES (64ull << 20)
+AKNN_API int aknn_forward_noalloc(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
+                                  const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
+                                  void* dists, int64_t* idxs, aknn_cache* cache, void* stream);
+
 /* Neighbour search only (idxs + dists).  Used by the aggregation-free paths. */
 AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
              const void* sigma, const void* R, void* dists, int64_t* idxs, void* workspace,
