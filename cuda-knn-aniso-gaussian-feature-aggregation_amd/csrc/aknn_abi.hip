@@ -575,6 +575,70 @@ int aknn_forward_noalloc(const aknn_dims* d, const aknn_tune* tune, const void* 
   return forward_run(d, &t, q, p, f, sigma, R, f_out, w_out, dists, idxs, ws, kScratchBytes, cache, stream);
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched copies
+namespace {
+struct CopyArgs {
+  aknn_copy_item it[AKNN_COPY_MAX_ITEMS];
+};
+
+// blockIdx.y = item; the blocks of an item stride over it in 16-byte chunks when both
+// ends are 16-byte aligned, then over the remaining bytes one at a time.  Every access
+// stays inside [0, bytes) of its item.
+__global__ __launch_bounds__(256) void copy_items_kernel(CopyArgs a) {
+  const aknn_copy_item it = a.it[blockIdx.y];
+  char* __restrict__ d = static_cast<char*>(it.dst);
+  const char* __restrict__ s = static_cast<const char*>(it.src);
+  const uint64_t n = it.bytes;
+  const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t done = 0;
+  if ((reinterpret_cast<uintptr_t>(d) | reinterpret_cast<uintptr_t>(s)) % 16 == 0) {
+    const uint64_t n16 = n / 16;
+    for (uint64_t k = tid; k < n16; k += stride)
+      reinterpret_cast<uint4*>(d)[k] = reinterpret_cast<const uint4*>(s)[k];
+    done = n16 * 16;
+  }
+  for (uint64_t k = done + tid; k < n; k += stride) d[k] = s[k];
+}
+}  // namespace
+
+extern "C" {
+
+int aknn_copy(const aknn_copy_item* items, int n_items, void* stream) {
+  if (n_items < 0 || n_items > AKNN_COPY_MAX_ITEMS)
+    return fail(AKNN_E_INVALID, "aknn_copy: 0 to %d items per call (got %d)", AKNN_COPY_MAX_ITEMS, n_items);
+  if (n_items == 0) return AKNN_OK;
+  if (!items) return fail(AKNN_E_INVALID, "aknn_copy: items is NULL");
+  CopyArgs a{};
+  uint64_t most = 0;
+  for (int i = 0; i < n_items; ++i) {
+    if (items[i].bytes && (!items[i].dst || !items[i].src)) return fail(AKNN_E_INVALID, "aknn_copy: null pointer");
+    a.it[i] = items[i];
+    most = std::max<uint64_t>(most, items[i].bytes);
+  }
+  if (most == 0) return AKNN_OK;
+  const uint64_t chunks = (most + 15) / 16;
+  const unsigned bx = static_cast<unsigned>(std::min<uint64_t>(1024, std::max<uint64_t>(1, (chunks + 255) / 256)));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  {
+    Span sp("copy", st);
+    hipLaunchKernelGGL(copy_items_kernel, dim3(bx, n_items), dim3(256), 0, st, a);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "copy");
+  return AKNN_OK;
+}
+
+int aknn_host_device_pointer(void* host, void** device) {
+  if (!host || !device) return fail(AKNN_E_INVALID, "aknn_host_device_pointer: null pointer");
+  *device = nullptr;
+  const hipError_t e = hipHostGetDevicePointer(device, host, 0);
+  if (e != hipSuccess) return hip_fail(e, "hipHostGetDevicePointer (memory not from hipHostMalloc?)");
+  return AKNN_OK;
+}
+
 int aknn_aggregate(const aknn_dims* d, const void* q, const void* p, const void* f,
                    const void* sigma, const void* R, const int64_t* idxs, void* f_out,
                    void* w_out, void* dists, void* stream) {

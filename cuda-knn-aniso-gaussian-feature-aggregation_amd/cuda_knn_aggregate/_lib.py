@@ -62,6 +62,8 @@ EXPORTS = (
     "aknn_iso_forward",
     "aknn_point_dots",
     "aknn_forward_noalloc",
+    "aknn_copy",
+    "aknn_host_device_pointer",
 )
 
 
@@ -88,6 +90,15 @@ class Cache(ctypes.Structure):
 
     _fields_ = [("query_order", ctypes.c_void_p), ("point_order", ctypes.c_void_p),
                 ("point_records", ctypes.c_void_p), ("point_dots", ctypes.c_void_p)]
+
+
+class CopyItem(ctypes.Structure):
+    """``aknn_copy_item`` (include/aknn.h)."""
+
+    _fields_ = [("dst", ctypes.c_void_p), ("src", ctypes.c_void_p), ("bytes", ctypes.c_uint64)]
+
+
+COPY_MAX_ITEMS = 8
 
 
 class Tune(ctypes.Structure):
@@ -136,6 +147,8 @@ _SIGS = {
     "aknn_iso_forward": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 4 + [_P] * 4 + [_P, ctypes.c_size_t, _P]),
     "aknn_point_dots": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P]),
     "aknn_forward_noalloc": (ctypes.c_int, [_DIMS, _TUNE] + [_P] * 9 + [_CACHE, _P]),
+    "aknn_copy": (ctypes.c_int, [ctypes.POINTER(CopyItem), ctypes.c_int, _P]),
+    "aknn_host_device_pointer": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p)]),
 }
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -168,6 +181,23 @@ def check(rc: int) -> None:
     if rc != AKNN_OK:
         msg = load().aknn_last_error()
         raise RuntimeError((msg.decode() if msg else "libaknn error") + f" (code {rc})")
+
+
+def host_device_pointer(t) -> int:
+    """Device address of a pinned host tensor (``pin_memory()``), for :func:`copy`."""
+    out = ctypes.c_void_p()
+    check(load().aknn_host_device_pointer(ctypes.c_void_p(t.data_ptr()), ctypes.byref(out)))
+    return out.value
+
+
+def copy(items, stream) -> None:
+    """Kernel-based copies ``[(dst_address, src_address, bytes), ...]`` on ``stream`` (a
+    hipStream_t as int); either side may be device memory or mapped pinned host memory."""
+    lib = load()
+    for s0 in range(0, len(items), COPY_MAX_ITEMS):
+        part = items[s0:s0 + COPY_MAX_ITEMS]
+        arr = (CopyItem * len(part))(*[CopyItem(d, s, n) for d, s, n in part])
+        check(lib.aknn_copy(arr, len(part), stream))
 
 
 def ptr(t) -> int | None:

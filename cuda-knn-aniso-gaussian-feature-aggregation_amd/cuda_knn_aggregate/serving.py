@@ -14,9 +14,10 @@ Scheduling (one host thread, non-blocking):
   the in-flight HBM estimate stays under ``hbm_budget`` (admission control);
 * a batch is one ragged launch sequence (search + aggregation + dq, every cloud of the
   batch in the same launches, its queries grouped by cloud) on its own HIP stream, in
-  buffers preallocated for that stream slot, so batches overlap on the device; results
-  come back with an asynchronous device-to-host copy into pinned staging and a
-  completion event;
+  buffers preallocated for that stream slot, so batches overlap on the device; the
+  queries go in and the results come back through the slot's pinned staging buffers,
+  moved by one copy kernel each way (no runtime copy calls on the scheduling thread,
+  where they were seen to block for milliseconds), with a completion event;
 * completed batches release their slot and budget at once, so new arrivals are admitted
   while other work is still on the GPU.
 
@@ -55,6 +56,17 @@ class Request:
         return self.done - self.arrival
 
 
+def _runs(rows):
+    """Merge (start, length) row ranges into maximal contiguous runs."""
+    out = []
+    for a, m in sorted(rows):
+        if out and out[-1][0] + out[-1][1] == a:
+            out[-1] = (out[-1][0], out[-1][1] + m)
+        else:
+            out.append((a, m))
+    return out
+
+
 @dataclass
 class _Batch:
     reqs: List[Request]
@@ -66,7 +78,8 @@ class _Batch:
 
 class _Slot:
     """One stream slot: its HIP stream and the buffers of a batch of up to ``max_q``
-    queries (device outputs and workspace, query offsets, pinned host staging)."""
+    queries (device outputs and workspace, query offsets, pinned host staging with the
+    device addresses the copy kernel uses)."""
 
     def __init__(self, device, dtype, F, K, max_q, n_clouds, ws_bytes):
         self.stream = torch.cuda.Stream(device=device)
@@ -77,10 +90,14 @@ class _Slot:
         self.idxs = torch.empty(max_q, K, dtype=torch.int64, device=device)
         self.dq = torch.empty(max_q, 3, dtype=dtype, device=device)
         self.ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=device)
-        self.q_off_host = torch.zeros(n_clouds + 1, dtype=torch.int64, pin_memory=True)
         self.q_off = torch.zeros(n_clouds + 1, dtype=torch.int64, device=device)
+        self.q_host = torch.empty(max_q, 3, dtype=dtype, pin_memory=True)
+        self.q_off_host = torch.zeros(n_clouds + 1, dtype=torch.int64, pin_memory=True)
         self.f_host = torch.empty(max_q, F, dtype=dtype, pin_memory=True)
         self.n_host = torch.empty(max_q, 3, dtype=dtype, pin_memory=True)
+        with torch.cuda.device(device):
+            self.q_host_dev, self.q_off_host_dev, self.f_host_dev, self.n_host_dev = (
+                _lib.host_device_pointer(t) for t in (self.q_host, self.q_off_host, self.f_host, self.n_host))
         self.event = torch.cuda.Event()
 
 
@@ -185,6 +202,8 @@ class Server:
         if req.q.is_cuda:  # device queries: the slot stream must see their producer's work
             req.ready = torch.cuda.Event()
             req.ready.record(torch.cuda.current_stream(req.q.device))
+        elif not req.q.is_contiguous():
+            req.q = req.q.contiguous()  # copied into the staging as one block of rows
         self.waiting.append(req)
 
     def batch_bytes(self, nq: int) -> int:
@@ -207,18 +226,29 @@ class Server:
         for c in range(self.n_clouds):
             start[c + 1] = start[c] + counts[c]
         nq = start[-1]
-        sl.q_off_host.copy_(torch.tensor(start, dtype=torch.int64))
+        offs = (ctypes.c_int64 * len(start))(*start)
+        ctypes.memmove(sl.q_off_host.data_ptr(), offs, ctypes.sizeof(offs))
         fill = start[:-1]
-        with torch.cuda.stream(st):
-            sl.q_off.copy_(sl.q_off_host, non_blocking=True)
-            for r in reqs:
-                if r.ready is not None:
+        el = sl.q.element_size()
+        host_rows = []
+        for r in reqs:
+            m = r.q.shape[0]
+            r.started = t
+            r.row = fill[r.cloud]
+            fill[r.cloud] += m
+            if r.q.is_cuda:  # device queries: a device copy after their producer's work
+                with torch.cuda.stream(st):
                     st.wait_event(r.ready)
-                m = r.q.shape[0]
-                r.started = t
-                r.row = fill[r.cloud]
-                sl.q[r.row:r.row + m].copy_(r.q, non_blocking=True)
-                fill[r.cloud] += m
+                    sl.q[r.row:r.row + m].copy_(r.q, non_blocking=True)
+            else:  # host queries: into the slot's pinned staging by this thread (one memcpy)
+                ctypes.memmove(sl.q_host.data_ptr() + r.row * 3 * el, r.q.data_ptr(), m * 3 * el)
+                host_rows.append((r.row, m))
+        # one copy kernel brings the staged rows and the offsets in (the slot's staging is
+        # reused only after its previous batch completed)
+        items = [(sl.q_off.data_ptr(), sl.q_off_host_dev, (self.n_clouds + 1) * 8)]
+        for row, m in _runs(host_rows):
+            items.append((sl.q.data_ptr() + row * 3 * el, sl.q_host_dev + row * 3 * el, m * 3 * el))
+        _lib.copy(items, st.cuda_stream)
         # forward and the query-side dq through the C ABI on the slot's stream, into the
         # slot's buffers (inputs validated at construction and submission)
         lib = self._lib
@@ -236,10 +266,9 @@ class Server:
                                      sl.q.data_ptr(), self.p.data_ptr(), self.f.data_ptr(), self.sigma.data_ptr(),
                                      self.R.data_ptr(), sl.idxs.data_ptr(), None, None, ctypes.byref(self.dq_cache),
                                      None, 0, sl.dq.data_ptr(), None, None, None, None, hs))
-        with torch.cuda.stream(st):
-            sl.f_host[:nq].copy_(sl.f_out[:nq], non_blocking=True)
-            sl.n_host[:nq].copy_(sl.dq[:nq], non_blocking=True)
-            sl.event.record(st)
+        _lib.copy([(sl.f_host_dev, sl.f_out.data_ptr(), nq * self.F * el),
+                   (sl.n_host_dev, sl.dq.data_ptr(), nq * 3 * el)], hs)
+        sl.event.record(st)
         self.inflight.append(_Batch(reqs, sl.event, slot, nbytes, nq))
         self.in_use += nbytes
         self.batches += 1

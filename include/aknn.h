@@ -199,6 +199,22 @@ AKNN_API int aknn_forward_noalloc(const aknn_dims* dims, const aknn_tune* tune, 
                                   const void* f, const void* sigma, const void* R, void* f_out, void* w_out,
                                   void* dists, int64_t* idxs, aknn_cache* cache, void* stream);
 
+/* Batched copies by a kernel (no counterpart in the reference; for a serving loop that
+ * moves query rows in and results out without the runtime's copy engine path).  Each
+ * item copies `bytes` from src to dst; either side may be device memory or mapped pinned
+ * host memory (the device address from aknn_host_device_pointer).  One launch on
+ * `stream` for up to AKNN_COPY_MAX_ITEMS items; items must not overlap. */
+typedef struct aknn_copy_item {
+  void* dst;
+  const void* src;
+  uint64_t bytes;
+} aknn_copy_item;
+#define AKNN_COPY_MAX_ITEMS 8
+AKNN_API int aknn_copy(const aknn_copy_item* items, int n_items, void* stream);
+/* Device address of pinned host memory allocated with hipHostMalloc (such as torch's
+ * pin_memory tensors), for aknn_copy. */
+AKNN_API int aknn_host_device_pointer(void* host, void** device);
+
 /* Neighbour search only (idxs + dists).  Used by the aggregation-free paths. */
 AKNN_API int aknn_knn(const aknn_dims* dims, const aknn_tune* tune, const void* q, const void* p,
              const void* sigma, const void* R, void* dists, int64_t* idxs, void* workspace,
