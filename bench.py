@@ -235,6 +235,54 @@ def eikonal_step_fn(params, K, up):
     return step
 
 
+def captured_step_ms(step, steps, warm=2):
+    """Capture ``step`` (library launches and elementwise torch ops only) in a HIP graph and
+    time its replays; returns ms per replay."""
+    import torch
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(warm):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    graph.replay()
+    torch.cuda.synchronize()
+    t = _timed(graph.replay, steps, 1)
+    del graph
+    return t * 1e3
+
+
+def _loss_grad(dq):
+    """h = d/d dq of sum (|dq| - 1)^2 by elementwise ops (capturable)."""
+    import torch
+
+    n = torch.sqrt(dq[..., 0:1] * dq[..., 0:1] + dq[..., 1:2] * dq[..., 1:2] + dq[..., 2:3] * dq[..., 2:3])
+    return 2.0 * (n - 1.0) * dq / n
+
+
+def explicit_step_fn(x, K, v, cache):
+    """The Eikonal step's gradients without autograd: forward, dq-only first order, the loss
+    gradient h, and the second order of <h, dq> (the same launches the autograd step makes
+    for these gradients), for capture."""
+    from cuda_knn_aggregate import _C
+
+    def step():
+        q, p, f, s, R = x
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, K, cache=cache)
+        up = v.expand_as(f_out)
+        dq = _C.knn_aggregate_aniso_backward(up, None, q, p, f, s, R, None, idxs, cache=cache, check=False,
+                                             needs=(True, False, False, False, False))[0]
+        return _C.knn_aggregate_aniso_backward_2nd(_loss_grad(dq), up, None, q, p, f, s, R, None, idxs, cache=cache,
+                                                   check=False, needs=(False, True, True, True, True, True))
+
+    return step
+
+
 def c1_times(steps=50, warm=5):
     """The reference's published shape (B=2, Q=12288, P=4096, K=16, F=64; README.md:47-55):
     forward, forward + first-order backward (upstream N(0,1) per query), and the Eikonal
@@ -304,10 +352,29 @@ def ragged_summary(steps=10, warm=2, loop_steps=3):
         tb = _timed(batched, steps, warm)
     tl = _timed(per_cloud, loop_steps, 1)
     kernels = {k: {"launches": n, "avg_ms": tot / n} for k, (n, tot) in tr.summary().items()}
+
+    # the same gradients as one captured graph (forward, dq, loss gradient, second order)
+    from cuda_knn_aggregate import ragged as rg
+
+    x = [t.detach() for t in params]
+    cache = layout.new_cache(x[0].dtype)
+
+    def explicit():
+        q, p, f, s, R = x
+        f_out, _, _, idxs = rg.forward(layout, q, p, f, s, R, r.K, tune=tune, cache=cache)
+        up = v.expand_as(f_out)
+        dq = rg.backward(layout, up, q, p, f, s, R, idxs, cache=cache, needs=(True, False, False, False, False))[0]
+        return rg.backward2(layout, _loss_grad(dq), up, q, p, f, s, R, idxs, cache=cache,
+                            needs=(False, True, True, True, True, True))
+
+    tg = captured_step_ms(explicit, steps)
     return {"workload": WORKLOADS["ragged"], "B": r.B, "Q_total": layout.Q, "P_total": layout.P, "K": r.K,
             "F": int(r.f.shape[1]), "value": layout.Q / tb, "unit": "queries/s", "ms_per_step": tb * 1e3,
             "steps": steps, "per_cloud_loop": {"value": layout.Q / tl, "ms_per_step": tl * 1e3},
-            "batched_over_loop": tl / tb, "tune": tune, "kernels": kernels}
+            "batched_over_loop": tl / tb,
+            "captured": {"ms_per_step": tg, "value": layout.Q / (tg * 1e-3), "over_loop": tl * 1e3 / tg,
+                         "step": "one HIP graph: ragged forward, dq, loss gradient, second order"},
+            "tune": tune, "kernels": kernels}
 
 
 def _serving_setup():
@@ -574,6 +641,14 @@ def main():
         "search_stats": search_stats,
     }
     if not args.no_extras:
+        # the step's gradients as one captured HIP graph (no autograd dispatch between launches)
+        print("[bench] captured step", flush=True)
+        xs = [t.detach() for t in params]
+        cache = _C.new_cache(B, Q, P, xs[0].dtype, dev)
+        tgraph = captured_step_ms(explicit_step_fn(xs, K, b.v, cache), args.steps)
+        out["captured_step"] = {"ms_per_step": tgraph, "value": B * Q / (tgraph * 1e-3), "unit": "queries/s",
+                                "step": "one HIP graph: forward, dq (v broadcast), loss gradient, second order"}
+        del cache
         # the same step with a general per-query upstream (materialised N(0,1) rows)
         print("[bench] general-upstream step", flush=True)
         g_up = torch.randn(B, Q, F, generator=torch.Generator().manual_seed(7)).to(dev)
