@@ -23,6 +23,7 @@
 // lmin comes from the closed-form 3x3 symmetric eigenvalues in FP64 minus a margin, the
 // result is scaled by (1 - 1e-6) to absorb the FP32 rounding of the tile test itself.
 // Points with any negative scale, or non-finite data, mark their tile "always scan".
+#include "aknn_bounds.h"
 #include "aknn_common.h"
 #include "aknn_internal.h"
 
@@ -76,15 +77,6 @@ struct OrdKey<double> {
     return __longlong_as_double(static_cast<long long>((k >> 63) ? (k & ~(1ull << 63)) : ~k));
   }
 };
-
-// Neighbouring floats (IEEE bit stepping).
-__device__ __forceinline__ float next_down(float f) {
-  if (f != f || f == -__builtin_inff()) return f;
-  if (f == 0.f) return __int_as_float(static_cast<int>(0x80000001u));
-  const int b = __float_as_int(f);
-  return __int_as_float(f > 0.f ? b - 1 : b + 1);
-}
-__device__ __forceinline__ float next_up(float f) { return -next_down(-f); }
 
 // ------------------------------------------------------------------ Morton codes
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -191,50 +183,8 @@ __global__ void codes_kernel(const T* __restrict__ xyz, Clouds cl, int64_t rows,
 }
 
 // ------------------------------------------------------------------ point bound
-// Certified lower bound mu_n of key(q, n) / |q - p_n|^2 (see the file comment);
-// negative means "no bound" (always scan).
-template <typename T, int KEY>
-__device__ double point_mu(const T* r9, const T* s3) {
-  constexpr double u = sizeof(T) == 4 ? 5.9604644775390625e-08 : 1.1102230246251565e-16;
-  constexpr double g = 4.0 * u / (1.0 - 4.0 * u);
-  if (KEY == AKNN_KEY_EUCLIDEAN) return (1.0 - g) * (1.0 - 2.0 * g * 1.7320508075688772);
-  double R[9], s[3];
-  for (int i = 0; i < 9; ++i) R[i] = static_cast<double>(r9[i]);
-  for (int i = 0; i < 3; ++i) s[i] = static_cast<double>(s3[i]);
-  for (int i = 0; i < 9; ++i)
-    if (!__builtin_isfinite(R[i])) return -1.0;
-  for (int i = 0; i < 3; ++i)
-    if (!(s[i] >= 0.0) || !__builtin_isfinite(s[i])) return -1.0;
-  // M = R^T diag(s) R (symmetric 3x3)
-  double M[3][3];
-  for (int a = 0; a < 3; ++a)
-    for (int c = 0; c < 3; ++c)
-      M[a][c] = s[0] * R[a] * R[c] + s[1] * R[3 + a] * R[3 + c] + s[2] * R[6 + a] * R[6 + c];
-  const double S = M[0][0] + M[1][1] + M[2][2];
-  const double m = S / 3.0;
-  const double p1 = M[0][1] * M[0][1] + M[0][2] * M[0][2] + M[1][2] * M[1][2];
-  const double a0 = M[0][0] - m, a1 = M[1][1] - m, a2 = M[2][2] - m;
-  const double p2 = a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * p1;
-  double lmin;
-  if (p2 <= 0.0) {
-    lmin = m;
-  } else {
-    const double pp = sqrt(p2 / 6.0);
-    const double b00 = a0 / pp, b11 = a1 / pp, b22 = a2 / pp;
-    const double b01 = M[0][1] / pp, b02 = M[0][2] / pp, b12 = M[1][2] / pp;
-    const double det = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
-                       b02 * (b01 * b12 - b11 * b02);
-    double rr = det / 2.0;
-    rr = rr < -1.0 ? -1.0 : (rr > 1.0 ? 1.0 : rr);
-    const double phi = acos(rr) / 3.0;
-    lmin = m + 2.0 * pp * cos(phi + 2.0943951023931957);  // smallest eigenvalue
-  }
-  // closed-form eigenvalues are accurate to a few ulp of the largest one
-  lmin -= 1e-12 * (fabs(m) + S);
-  if (!(lmin > 0.0)) return 0.0;
-  const double mu = (1.0 - g) * (lmin - 2.0 * g * sqrt(lmin * S));
-  return mu > 0.0 ? mu : 0.0;
-}
+// point_mu, stored_mu and box_needed: aknn_bounds.h (shared with the host test of the
+// certificate).
 
 // ------------------------------------------------------------------ tile build
 // One wave per tile.  rec: [tiles*kTile][16] records; hdr: [tiles][8] =
@@ -288,7 +238,7 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
       lo3[c] = static_cast<double>(f) <= static_cast<double>(v[c]) ? f : next_down(f);
       hi3[c] = static_cast<double>(f) >= static_cast<double>(v[c]) ? f : next_up(f);
     }
-    mu = point_mu<T, KEY>(v + 3, v + 12);
+    mu = point_mu<T, KEY == AKNN_KEY_EUCLIDEAN>(v + 3, v + 12);
     if (!(mu >= 0.0)) bad = true;
   } else {
     // padding record: NaN position, so every key evaluated against it is NaN and it
@@ -304,13 +254,7 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
       h[c] = lo3[c];
       h[3 + c] = hi3[c];
     }
-    float muf = -1.f;
-    if (!bad) {
-      const double m = mu * (1.0 - 1e-6);
-      muf = static_cast<float>(m);
-      if (static_cast<double>(muf) > m) muf = next_down(muf);
-    }
-    h[6] = muf;
+    h[6] = stored_mu(mu, bad);
   };
   for (int off = 1; off < 64; off <<= 1) {
     for (int c = 0; c < 3; ++c) {
@@ -465,20 +409,6 @@ template <typename T>
 __device__ __forceinline__ int rec_index(T v) {
   if constexpr (sizeof(T) == 4) return __float_as_int(v);
   else return static_cast<int>(__double_as_longlong(v));
-}
-
-// Lower bound of every key of a box's points for this lane's query, in the key's
-// precision (the box holds the exact coordinates, so each difference rounds relative
-// to itself), and whether the lane still needs the box.
-template <typename T>
-__device__ __forceinline__ bool box_needed(const float* __restrict__ h, T qx, T qy, T qz, T thr,
-                                           bool valid) {
-  const T mu = static_cast<T>(h[6]);
-  const T dx = fmax(fmax(static_cast<T>(h[0]) - qx, qx - static_cast<T>(h[3])), T(0));
-  const T dy = fmax(fmax(static_cast<T>(h[1]) - qy, qy - static_cast<T>(h[4])), T(0));
-  const T dz = fmax(fmax(static_cast<T>(h[2]) - qz, qz - static_cast<T>(h[5])), T(0));
-  const T lb = mu * (dx * dx + dy * dy + dz * dz);
-  return valid & ((mu < T(0)) | !(lb > thr));
 }
 
 // Outward walk over [lo, hi) from c: c, c+1, c-1, c+2, c-2, ... (each index once).
@@ -958,11 +888,7 @@ template <typename T>
 __device__ __forceinline__ T box_bound(const float* __restrict__ h, T qx, T qy, T qz) {
   const float4 a = *reinterpret_cast<const float4*>(h);
   const float4 c = *reinterpret_cast<const float4*>(h + 4);
-  const T mu = static_cast<T>(c.z);
-  const T dx = fmax(fmax(static_cast<T>(a.x) - qx, qx - static_cast<T>(a.w)), T(0));
-  const T dy = fmax(fmax(static_cast<T>(a.y) - qy, qy - static_cast<T>(c.x)), T(0));
-  const T dz = fmax(fmax(static_cast<T>(a.z) - qz, qz - static_cast<T>(c.y)), T(0));
-  return mu < T(0) ? -__builtin_inf() : mu * (dx * dx + dy * dy + dz * dz);
+  return c.z < 0.f ? -__builtin_inf() : box_lower_bound<T>(a.x, a.y, a.z, a.w, c.x, c.y, c.z, qx, qy, qz);
 }
 
 template <typename T>
