@@ -200,6 +200,31 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 constexpr int64_t kSplitBlocks = 16384;
 
+// Super-tiles of the batch at `per` tiles each.
+int64_t super_count(const aknn_dims* d, int64_t per) {
+  if (!ragged(d)) return d->B * ceil_div(ceil_div(d->P, aknn::kTile), per);
+  int64_t n = 0;
+  for (int64_t b = 0; b < d->B; ++b) n += ceil_div(ceil_div(d->p_off_host[b + 1] - d->p_off_host[b], aknn::kTile), per);
+  return n;
+}
+
+void set_super(aknn::PrunedLaunch& L, const aknn_dims* d, int per) {
+  L.super_tiles = per;
+  L.nsuper = super_count(d, per);
+}
+
+// Tile geometry and occupancy from a tune (results never depend on them).  A prebuilt
+// point index fixes the defaults it was built with; sub-tile sizes and occupancy
+// targets other than the defaults exist for the FP32 search with 9 to 16 neighbours.
+void apply_geometry(aknn::PrunedLaunch& L, const aknn_dims* d, const aknn_tune* t) {
+  if (!t || t->index) return;
+  if (t->super_tiles == 8 || t->super_tiles == 16 || t->super_tiles == 32) set_super(L, d, t->super_tiles);
+  const bool variants = d->dtype == AKNN_F32 && d->K > 8 && d->K <= 16;
+  if (variants && (t->sub_records == 4 || t->sub_records == 8 || t->sub_records == 16)) L.sub_rec = t->sub_records;
+  if (variants && L.sub_rec == aknn::kSubRecDefault && (t->waves_per_eu == 4 || t->waves_per_eu == 8))
+    L.waves = t->waves_per_eu;
+}
+
 aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   aknn::PrunedLaunch L{};
   L.dtype = d->dtype;
@@ -213,16 +238,18 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
       const int64_t pb = d->p_off_host[b + 1] - d->p_off_host[b];
       const int64_t tiles = ceil_div(pb, aknn::kTile);
       L.ntiles += tiles;
-      L.nsuper += ceil_div(tiles, aknn::kSuper);
       L.nqtiles += ceil_div(d->q_off_host[b + 1] - d->q_off_host[b], aknn::kQueryTile);
       L.max_points = std::max(L.max_points, pb);
     }
   } else {
     L.ntiles = d->B * ceil_div(d->P, aknn::kTile);
-    L.nsuper = d->B * ceil_div(ceil_div(d->P, aknn::kTile), aknn::kSuper);
     L.nqtiles = d->B * ceil_div(d->Q, aknn::kQueryTile);
     L.max_points = d->P;
   }
+  L.sub_rec = aknn::kSubRecDefault;
+  L.waves = 0;
+  L.nsuper_alloc = super_count(d, aknn::kSuperMin);
+  set_super(L, d, aknn::kSuper);
   // Few query tiles (small or scattered query sets) cannot fill the device and the
   // slowest tile's walk sets the kernel time: split every tile's walk over up to 64
   // blocks with at least 8 point tiles per block and cloud, and at most kSplitBlocks
@@ -290,6 +317,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.stats = tune ? static_cast<unsigned long long*>(tune->stats) : nullptr;
     PL.index = tune ? tune->index : nullptr;
     PL.wave = pick_wave(d, tune) ? 1 : 0;
+    apply_geometry(PL, d, tune);
     if (tune && tune->point_split > 0) {
       int w = 1;
       while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;

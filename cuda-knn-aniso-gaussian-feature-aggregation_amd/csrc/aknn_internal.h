@@ -22,9 +22,14 @@ struct KnnLaunch {
 };
 
 // Point tiles of the tiled search and the per-query walk: records per tile, and tiles
-// per super-tile.  Queries go in tiles of 64 (one per lane of a wave).
+// per super-tile (default; the tuner takes 8, 16 or 32).  Queries go in tiles of 64
+// (one per lane of a wave).  Sub-tiles of the tiled search: 8 records by default (the
+// tuner takes 4, 8 or 16).  The smallest sizes set the header arrays' sizes.
 constexpr int kTile = 64;
 constexpr int kSuper = 16;
+constexpr int kSuperMin = 8;
+constexpr int kSubRecDefault = 8;
+constexpr int kSubRecMin = 4;
 constexpr int kQueryTile = 64;
 
 // Tile-pruned top-K (same results as KnnLaunch's brute-force scan).
@@ -47,6 +52,10 @@ struct PrunedLaunch {
   size_t ws_bytes;
   const void* index;      // prebuilt point index (pruned_index_bytes), or null: built per call
   int32_t wave;           // per-query walk (one wave per query) instead of query tiles
+  int32_t super_tiles;    // tiles per super-tile (8, 16 or 32)
+  int32_t sub_rec;        // records per sub-tile of the tiled search (4, 8 or 16)
+  int32_t waves;          // tiled search occupancy target (waves per SIMD; 0 = the list's default)
+  int64_t nsuper_alloc;   // super-tiles at the smallest super-tile size (sizes the headers)
 };
 
 // Gather-side (query rows) work: forward aggregation, first- and second-order dq.

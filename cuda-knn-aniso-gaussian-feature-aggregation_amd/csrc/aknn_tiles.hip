@@ -29,8 +29,7 @@
 
 namespace aknn {
 
-constexpr int kSubRec = 8;                 // records per sub-tile
-constexpr int kSubs = kTile / kSubRec;     // sub-tiles per tile
+constexpr int kSubsMax = kTile / kSubRecMin;  // sub-tile headers kept per tile
 constexpr int kTileSentinel = kNoPoint;
 constexpr int kWaveSerialInserts = 16;  // per-query walk: more candidates in a tile -> sort
 
@@ -42,7 +41,7 @@ struct PrunedState {
   uint32_t *pperm, *qperm;
   int64_t *ptp, *qtp, *stp;
   void* rec;
-  float *hdr, *shdr, *sub;  // tile, super-tile and sub-tile (kSubRec records) headers
+  float *hdr, *shdr, *sub;  // tile, super-tile and sub-tile headers ([tiles][kSubsMax][8])
   int32_t* tcnt;
   void* tmp;
   size_t tmp_bytes;
@@ -189,9 +188,9 @@ __global__ void codes_kernel(const T* __restrict__ xyz, Clouds cl, int64_t rows,
 // ------------------------------------------------------------------ tile build
 // One wave per tile.  rec: [tiles*kTile][16] records; hdr: [tiles][8] =
 // {lo x y z, hi x y z, mu (float, rounded down; < 0 = always scan), first code (bits)};
-// sub: [tiles][kSubs][8] = the same box and mu (last value unused) of each kSubRec-record
-// sub-tile.
-template <typename T, int KEY>
+// sub: [tiles][kSubsMax][8], of which the first kTile / SUBREC hold the same box and mu
+// (last value unused) of each SUBREC-record sub-tile.
+template <typename T, int KEY, int SUBREC>
 __global__ __launch_bounds__(64) void tile_build_kernel(
     const T* __restrict__ p, const T* __restrict__ sigma, const T* __restrict__ R, Clouds cl,
     const int64_t* __restrict__ tile_prefix, const uint32_t* __restrict__ perm,
@@ -247,7 +246,7 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     if constexpr (sizeof(T) == 4) out[15] = __int_as_float(kTileSentinel);
     else out[15] = __longlong_as_double(static_cast<long long>(kTileSentinel));
   }
-  // Reductions over groups of kSubRec lanes (the sub-tiles), then over the wave (the tile).
+  // Reductions over groups of SUBREC lanes (the sub-tiles), then over the wave (the tile).
   // min / max are exact, so the grouping does not change any value.
   auto write_box = [&](float* h) {
     for (int c = 0; c < 3; ++c) {
@@ -263,8 +262,8 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     }
     mu = fmin(mu, __shfl_xor(mu, off, 64));
     bad = bad | (__shfl_xor(static_cast<int>(bad), off, 64) != 0);
-    if (off == kSubRec / 2 && (lane % kSubRec) == 0) {
-      float* h = sub + (t * kSubs + lane / kSubRec) * 8;
+    if (off == SUBREC / 2 && (lane % SUBREC) == 0) {
+      float* h = sub + (t * kSubsMax + lane / SUBREC) * 8;
       write_box(h);
       h[7] = 0.f;
     }
@@ -430,10 +429,11 @@ constexpr int search_waves() {
 }
 
 // grid.x = query tiles of 64 sorted queries (ragged: via the query tile prefix),
-// grid.y = clouds (dense).  Two-level walk: super-tiles of kSuper tiles, then tiles.
-template <typename T, int KMAX, int KEY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves<T, KMAX>(),
-                                                                   search_waves<T, KMAX>()))) void knn_pruned_kernel(
+// grid.y = clouds (dense).  Two-level walk: super-tiles of `super` tiles, then tiles,
+// then SUBREC-record sub-tiles.  WAVES: occupancy target (0 = the list's default).
+template <typename T, int KMAX, int KEY, int SUBREC, int WAVES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : search_waves<T, KMAX>(),
+                                                                   WAVES ? WAVES : search_waves<T, KMAX>()))) void knn_pruned_kernel(
     const T* __restrict__ q, Clouds cl, const int64_t* __restrict__ qtiles,
     const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys,
     const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
@@ -441,8 +441,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     const float* __restrict__ sub, const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists,
     int64_t* __restrict__ idxs, unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
     int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd, const T* __restrict__ pts,
-    const T* __restrict__ sig, const T* __restrict__ rot) {
+    const T* __restrict__ sig, const T* __restrict__ rot, int super) {
   using Ord = OrdKey<T>;
+  constexpr int kSubs = kTile / SUBREC;
   const int lane = threadIdx.x;
   int64_t b, q0;
   // XCD-aware order of the (spatially sorted) query tiles: each XCD's L2 then holds
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
     if (__float_as_uint(hdr[static_cast<int64_t>(mid) * 8 + 7]) <= code0) lo = mid; else hi = mid - 1;
   }
   const int seed_t = lo;
-  const int seed_s = sb + (seed_t - tb) / kSuper;
+  const int seed_s = sb + (seed_t - tb) / super;
   const int iway = static_cast<int>(way);
 
   // Split walk (ways > 1): this block owns the tiles t with (t - tb) % ways == way and
@@ -523,8 +524,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
       thr = fmin(thr, Ord::dec(__hip_atomic_load(bnd + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
     if (!__any(box_needed<T>(shdr + static_cast<int64_t>(s) * 8, qx, qy, qz, fmin(L.last(), thr), valid)))
       continue;
-    const int t0 = tb + (s - sb) * kSuper;
-    const int t1 = min(te, t0 + kSuper);
+    const int t0 = tb + (s - sb) * super;
+    const int t1 = min(te, t0 + super);
     Outward to(t0, t1, s == seed_s ? seed_t : t0);
     for (int tstep = 0; tstep < t1 - t0; ++tstep) {
       const int t = to.next(tstep);
@@ -534,17 +535,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
         continue;
       ++n_scan;
       const int cnt = tcnt[t];
-      // the tile's sub-tiles of kSubRec records: scan those some lane still needs
-      const float* __restrict__ sh = sub + static_cast<int64_t>(t) * (kSubs * 8);
+      // the tile's sub-tiles of SUBREC records: scan those some lane still needs
+      const float* __restrict__ sh = sub + static_cast<int64_t>(t) * (kSubsMax * 8);
       // lim = fmin(last kept key, thr): thr is fixed within a tile, so lim changes only
       // with an insertion
       T lim = fmin(L.last(), thr);
       unsigned m = 0;
 #pragma unroll
       for (int u = 0; u < kSubs; ++u)
-        if ((kSubRec * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, lim, valid))) m |= 1u << u;
+        if ((SUBREC * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, lim, valid))) m |= 1u << u;
       if (m == 0) continue;
-      const int total = kSubRec * __builtin_popcount(m);
+      const int total = SUBREC * __builtin_popcount(m);
       n_pts += total;
       const T* __restrict__ base = rec + static_cast<int64_t>(t) * (kTile * 16);
       // The needed sub-tiles' records a pair at a time, as one stream; the next pair's
@@ -554,16 +555,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
       // The pairs of a sub-tile are unrolled, so their offsets are constants and the
       // stream's bookkeeping runs once per sub-tile.
       unsigned rest = m;
-      const T* cur = base + __builtin_ctz(rest) * (kSubRec * 16);
+      const T* cur = base + __builtin_ctz(rest) * (SUBREC * 16);
       rest &= rest - 1;
       Rec16<T> a0 = load_rec16(cur);
       Rec16<T> a1 = load_rec16(cur + 16);
       while (true) {
         // the next sub-tile (past the end: the current one again, loaded, not evaluated)
-        const T* nxt = rest ? base + __builtin_ctz(rest) * (kSubRec * 16) : cur;
+        const T* nxt = rest ? base + __builtin_ctz(rest) * (SUBREC * 16) : cur;
 #pragma unroll
-        for (int j = 0; j < kSubRec; j += 2) {
-          const T* nx = j + 2 < kSubRec ? cur + (j + 2) * 16 : nxt;
+        for (int j = 0; j < SUBREC; j += 2) {
+          const T* nx = j + 2 < SUBREC ? cur + (j + 2) * 16 : nxt;
           const Rec16<T> b0 = load_rec16(nx);
           const Rec16<T> b1 = load_rec16(nx + 16);
           const T c0 = rec_key<T, KEY>(qx, qy, qz, a0);
@@ -643,10 +644,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(search_waves
 }
 
 // Super-tile headers: box union and min mu (negative if any tile is "always scan")
-// of kSuper consecutive tiles of one cloud.  One thread per super-tile.
+// of `super` consecutive tiles of one cloud.  One thread per super-tile.
 __global__ void super_build_kernel(Clouds cl, const int64_t* __restrict__ ptiles,
                                    const int64_t* __restrict__ stiles, int64_t nsuper,
-                                   const float* __restrict__ hdr, float* __restrict__ shdr) {
+                                   const float* __restrict__ hdr, float* __restrict__ shdr, int super) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= nsuper) return;
   int64_t b = 0, tb, te, sb;
@@ -666,8 +667,8 @@ __global__ void super_build_kernel(Clouds cl, const int64_t* __restrict__ ptiles
     te = tb + ptiles[1];
     sb = b * stiles[1];
   }
-  const int64_t t0 = tb + (s - sb) * kSuper;
-  const int64_t t1 = min(te, t0 + kSuper);
+  const int64_t t0 = tb + (s - sb) * super;
+  const int64_t t1 = min(te, t0 + super);
   float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
   float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
   float mu = __builtin_inff();
@@ -972,7 +973,7 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
     const T* __restrict__ q, Clouds cl, int64_t qrows, const int64_t* __restrict__ ptiles,
     const int64_t* __restrict__ stiles, const T* __restrict__ rec, const float* __restrict__ hdr,
     const float* __restrict__ shdr, int K, T* __restrict__ dists, int64_t* __restrict__ idxs,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, int super) {
   const int lane = threadIdx.x & 63;
   // the wave's query (made wave-uniform, so the walk's bookkeeping is scalar)
   const int64_t row = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) +
@@ -1095,8 +1096,8 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(
       const int sl = wave_argmin<T>(slb, ssel, inf_v);
       if (lane == sl) sdone = true;
       ++n_super;
-      const int64_t t0 = tb + (c * 64 + sl) * kSuper;
-      const int64_t nt = te - t0 < kSuper ? te - t0 : kSuper;
+      const int64_t t0 = tb + (c * 64 + sl) * super;
+      const int64_t nt = te - t0 < super ? te - t0 : super;
       const bool tin = lane < nt;
       const T tlb = tin ? box_bound<T>(hdr + (t0 + lane) * 8, qx, qy, qz) : inf_v;
       bool tdone = !tin;
@@ -1141,8 +1142,20 @@ static hipError_t launch_wave(const PrunedLaunch& L, const PrunedState& S, hipSt
   if (L.qrows == 0) return hipSuccess;
   hipLaunchKernelGGL((knn_wave_kernel<T, KEY>), dim3(static_cast<unsigned>((L.qrows + 3) / 4)), dim3(256), 0,
                      st, static_cast<const T*>(L.q), L.cl, L.qrows, S.ptp, S.stp, static_cast<const T*>(S.rec),
-                     S.hdr, S.shdr, static_cast<int>(L.K), static_cast<T*>(L.dists), L.idxs, L.stats);
+                     S.hdr, S.shdr, static_cast<int>(L.K), static_cast<T*>(L.dists), L.idxs, L.stats,
+                     static_cast<int>(L.super_tiles));
   return hipGetLastError();
+}
+
+template <typename T, int KMAX, int KEY, int SUBREC, int WAVES>
+static void launch_search(const PrunedLaunch& L, const PrunedState& S, dim3 grid, int ways, hipStream_t st) {
+  using U = typename OrdKey<T>::U;
+  hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY, SUBREC, WAVES>), grid, dim3(64), 0, st,
+                     static_cast<const T*>(L.q), L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp,
+                     S.stp, static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
+                     static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
+                     static_cast<U*>(S.bnd), static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
+                     static_cast<const T*>(L.R), static_cast<int>(L.super_tiles));
 }
 
 template <typename T, int KMAX, int KEY>
@@ -1157,12 +1170,17 @@ static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, h
   dim3 grid;
   if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles * ways), 1);
   else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64 * ways), static_cast<unsigned>(L.cl.B));
-  hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY>), grid, dim3(64), 0, st, static_cast<const T*>(L.q),
-                     L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp, S.stp,
-                     static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
-                     static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
-                     static_cast<U*>(S.bnd), static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
-                     static_cast<const T*>(L.R));
+  // tile geometry and occupancy variants (the host admits them for FP32, 9..16 neighbours)
+  if constexpr (sizeof(T) == 4 && KMAX == 16) {
+    if (L.sub_rec == 4) launch_search<T, KMAX, KEY, 4, 0>(L, S, grid, ways, st);
+    else if (L.sub_rec == 16) launch_search<T, KMAX, KEY, 16, 0>(L, S, grid, ways, st);
+    else if (L.waves == 4) launch_search<T, KMAX, KEY, 8, 4>(L, S, grid, ways, st);
+    else if (L.waves == 8) launch_search<T, KMAX, KEY, 8, 8>(L, S, grid, ways, st);
+    else launch_search<T, KMAX, KEY, 8, 0>(L, S, grid, ways, st);
+  } else {
+    if (L.sub_rec != kSubRecDefault || L.waves != 0) return hipErrorInvalidValue;
+    launch_search<T, KMAX, KEY, 8, 0>(L, S, grid, ways, st);
+  }
   if (ways > 1) {
     const int per = 64 / ways;
     hipLaunchKernelGGL((pruned_merge_kernel<T, KMAX>), dim3(static_cast<unsigned>((L.qrows + per - 1) / per)),
@@ -1213,8 +1231,8 @@ static void carve_index(const PrunedLaunch& L, Carver& c, PrunedState& S) {
   S.stp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.rec = c.take(L.ntiles * kTile * 16 * el);
   S.hdr = reinterpret_cast<float*>(c.take(L.ntiles * 8 * 4));
-  S.shdr = reinterpret_cast<float*>(c.take(L.nsuper * 8 * 4));
-  S.sub = reinterpret_cast<float*>(c.take(L.ntiles * kSubs * 8 * 4));
+  S.shdr = reinterpret_cast<float*>(c.take(L.nsuper_alloc * 8 * 4));
+  S.sub = reinterpret_cast<float*>(c.take(L.ntiles * kSubsMax * 8 * 4));
   S.tcnt = reinterpret_cast<int32_t*>(c.take(L.ntiles * 4));
 }
 
@@ -1281,13 +1299,14 @@ __global__ void tile_prefix_kernel2(const int64_t* __restrict__ off, int64_t B, 
   out[B] = acc;
 }
 
-// super-tile prefix from the tile prefix (ragged)
-__global__ void super_prefix_kernel(const int64_t* __restrict__ ptp, int64_t B, int64_t* __restrict__ out) {
+// super-tile prefix from the tile prefix (ragged), `super` tiles per super-tile
+__global__ void super_prefix_kernel(const int64_t* __restrict__ ptp, int64_t B, int64_t super,
+                                    int64_t* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int64_t acc = 0;
   for (int64_t b = 0; b < B; ++b) {
     out[b] = acc;
-    acc += (ptp[b + 1] - ptp[b] + kSuper - 1) / kSuper;
+    acc += (ptp[b + 1] - ptp[b] + super - 1) / super;
   }
   out[B] = acc;
 }
@@ -1321,17 +1340,28 @@ static hipError_t prep_points(const PrunedLaunch& L, const PrunedState& S, hipSt
   if (e != hipSuccess) return e;
   if (L.cl.p_off) {
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.p_off, L.cl.B, int64_t(kTile), S.ptp);
-    hipLaunchKernelGGL(super_prefix_kernel, dim3(1), dim3(64), 0, st, S.ptp, L.cl.B, S.stp);
+    hipLaunchKernelGGL(super_prefix_kernel, dim3(1), dim3(64), 0, st, S.ptp, L.cl.B,
+                       static_cast<int64_t>(L.super_tiles), S.stp);
   } else {
     const int64_t per = (L.cl.P + kTile - 1) / kTile;
     hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, per, S.ptp);
-    hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, (per + kSuper - 1) / kSuper, S.stp);
+    hipLaunchKernelGGL(dense_prefix_kernel, dim3(1), dim3(64), 0, st, (per + L.super_tiles - 1) / L.super_tiles,
+                       S.stp);
   }
-  hipLaunchKernelGGL((tile_build_kernel<T, KEY>), dim3(static_cast<unsigned>(L.ntiles)), dim3(64), 0, st,
-                     p, static_cast<const T*>(L.sigma), static_cast<const T*>(L.R), L.cl, S.ptp, S.pperm,
-                     S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+  const dim3 tg(static_cast<unsigned>(L.ntiles));
+  const T* sg = static_cast<const T*>(L.sigma);
+  const T* rt = static_cast<const T*>(L.R);
+  if (L.sub_rec == 4)
+    hipLaunchKernelGGL((tile_build_kernel<T, KEY, 4>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+  else if (L.sub_rec == 16)
+    hipLaunchKernelGGL((tile_build_kernel<T, KEY, 16>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+  else
+    hipLaunchKernelGGL((tile_build_kernel<T, KEY, 8>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
   hipLaunchKernelGGL(super_build_kernel, dim3(static_cast<unsigned>((L.nsuper + bs - 1) / bs)), dim3(bs), 0, st,
-                     L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr);
+                     L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr, static_cast<int>(L.super_tiles));
   return hipGetLastError();
 }
 

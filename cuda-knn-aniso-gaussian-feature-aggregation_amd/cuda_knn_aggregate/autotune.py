@@ -1,11 +1,20 @@
 """Autotuner for the neighbour search launch.
 
-Tunables (``aknn_tune``): the search algorithm (brute-force scan or tiled search), and
-for the brute scan the queries held per lane (1, 2, 4 — register lists per lane) and
-the number of waves splitting a query tile's points (1..8 — their lists merge through
-LDS).  Every candidate returns the same neighbours; only the time differs.  The tuner
-times each candidate on the caller's own inputs (hipEvents on the current stream,
-median of a few runs), keeps the fastest per shape class in memory and in a JSON file
+Tunables (``aknn_tune``; every candidate returns the same neighbours, only the time
+differs):
+
+* the algorithm and its queries per wave: the tiled search (64 queries per wave, one per
+  lane), the per-query walk (one query per wave) or the brute-force scan (64 x 1, 2 or 4
+  queries per wave, register lists per lane);
+* point-tile geometry: records per sub-tile the tiled search tests and scans (4, 8, 16)
+  and tiles per super-tile (8, 16, 32, also for the per-query walk);
+* occupancy: the tiled search's waves per SIMD (4, 6 by default, 8);
+* LDS layout of the brute scan: waves splitting a query tile's points (1..8), whose
+  lists merge through LDS.
+
+The tuner times each candidate on the caller's own inputs (hipEvents on the current
+stream, median of a few runs; a candidate whose first run is far slower than the best so
+far is not timed further), keeps the fastest per shape class in memory and in a JSON file
 (``$AKNN_TUNE_CACHE`` or ``~/.cache/cuda_knn_aggregate/autotune.json``), and returns a
 ``tune`` dict accepted by every forward entry point.
 """
@@ -21,7 +30,9 @@ import torch
 from . import _C
 
 CANDIDATES = (
-    [{"search": "tiles"}, {"search": "wave"}]
+    [{"search": "tiles", "sub_records": r, "super_tiles": n} for r in (8, 4, 16) for n in (16, 8, 32)]
+    + [{"search": "tiles", "waves_per_eu": w} for w in (4, 8)]
+    + [{"search": "wave", "super_tiles": n} for n in (16, 8, 32)]
     + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]
 )
 
@@ -98,9 +109,16 @@ def tuned(q, p, sigma, R, K, *, key="aniso", layout=None, candidates=CANDIDATES,
         def run(t):
             rg.forward(layout, q, p, f1, sigma, R, K, key=key, tune=t)
     results = []
+    best = float("inf")
     for t in candidates:
         run(t)  # warm-up (code object load, allocator)
-        results.append((_time(lambda: run(t)), t))
+        first = _time(lambda: run(t), reps=1)
+        if first > 3.0 * best:  # far off the best: not worth its timed runs
+            results.append((first, t))
+            continue
+        ms = _time(lambda: run(t))
+        best = min(best, ms)
+        results.append((ms, t))
     results.sort(key=lambda x: x[0])
     best_ms, best = results[0]
     with _lock:

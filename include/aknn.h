@@ -116,13 +116,18 @@ typedef struct aknn_tune {
                                tiles: blocks sharing one query tile's walk (a power of
                                two, at most the library's own choice for the shape)    */
   int32_t search;           /* AKNN_SEARCH_*                                            */
-  int32_t reserved;
+  int32_t sub_records;      /* tiles: records per sub-tile the search tests and scans (4, 8
+                               or 16; FP32 with 9 to 16 neighbours, else 8)             */
   void* stats;              /* tiles: optional device uint64[5] the search adds its counts
                                to (super-tile tests, tile tests, tiles scanned, points
                                scanned, waves); NULL = off                               */
   const void* index;        /* optional point index from aknn_index_build for these
                                points (same dims except the queries): the tiled search
-                               then skips its per-call point preparation              */
+                               then skips its per-call point preparation (and keeps the
+                               default tile geometry the index was built with)         */
+  int32_t super_tiles;      /* tiles and per-query walk: tiles per super-tile (8, 16, 32) */
+  int32_t waves_per_eu;     /* tiles: occupancy target of the search (4 or 8 waves per
+                               SIMD; FP32 with 9 to 16 neighbours and 8-record sub-tiles) */
 } aknn_tune;
 
 /* Point index (no counterpart in the reference; for callers that keep clouds resident

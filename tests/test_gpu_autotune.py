@@ -1,0 +1,56 @@
+"""GPU tests of the search tunables and the autotuner: every candidate launch (algorithm,
+queries per wave, sub-tile records, super-tile size, occupancy target, brute-scan LDS
+split) returns the default launch's neighbours bit for bit, and the tuner's choice comes
+back from its cache file without timing again."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from cuda_knn_aggregate import _C, autotune, workloads
+from cuda_knn_aggregate import ragged as rg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("K", [16, 8, 12])
+def test_every_candidate_same_neighbours_dense(cuda, K):
+    b = workloads.clustered_batch(Q=20000, P=20000, K=K, F=4, seed=K).to(cuda)
+    ref = _C.knn_aniso(b.q, b.p, b.sigma, b.R, K)
+    for t in autotune.CANDIDATES:
+        got = _C.knn_aniso(b.q, b.p, b.sigma, b.R, K, tune=t)
+        assert torch.equal(got[1], ref[1]) and torch.equal(got[0], ref[0]), t
+    # and through the forward (aggregation on the same neighbours)
+    fo = _C.knn_aggregate_aniso_forward(b.q, b.p, b.f, b.sigma, b.R, K)
+    for t in autotune.CANDIDATES[:11]:
+        assert all(torch.equal(x, y) for x, y in
+                   zip(_C.knn_aggregate_aniso_forward(b.q, b.p, b.f, b.sigma, b.R, K, tune=t), fo)), t
+
+
+def test_every_candidate_same_neighbours_ragged(cuda):
+    r = workloads.ragged_batch(B=12, p_range=(2000, 20000), q_range=(500, 20000), K=16, F=4, seed=3)
+    layout = rg.RaggedLayout(r.q_off, r.p_off, cuda)
+    d = r.to(cuda)
+    ref = rg.forward(layout, d.q, d.p, d.f, d.sigma, d.R, r.K)
+    for t in autotune.CANDIDATES:
+        got = rg.forward(layout, d.q, d.p, d.f, d.sigma, d.R, r.K, tune=t)
+        assert all(torch.equal(x, y) for x, y in zip(got, ref)), t
+
+
+def test_tuner_cache_round_trip(cuda, tmp_path, monkeypatch):
+    monkeypatch.setenv("AKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_mem", {})
+    b = workloads.clustered_batch(Q=30000, P=16384, K=16, F=4, seed=5).to(cuda)
+    best = autotune.tuned(b.q, b.p, b.sigma, b.R, 16)
+    assert best in [dict(t) for t in autotune.CANDIDATES]
+    assert (tmp_path / "tune.json").exists()
+    monkeypatch.setattr(autotune, "_mem", {})
+
+    def no_timing(*a, **k):
+        raise AssertionError("timed again instead of reading the cache")
+
+    monkeypatch.setattr(autotune, "_time", no_timing)
+    assert autotune.tuned(b.q, b.p, b.sigma, b.R, 16) == best
+    ref = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16)
+    got = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune=best)
+    assert torch.equal(got[1], ref[1])
