@@ -219,7 +219,7 @@ def _timed(fn, steps, warm):
     return (time.perf_counter() - t0) / steps
 
 
-def eikonal_step_fn(params, K, up):
+def eikonal_step_fn(params, K, up, tune=None):
     """One fwd + 1st (create_graph) + 2nd-order step; ``up(f_out)`` is the upstream of the
     first order (the read-out v broadcast, or a per-query tensor)."""
     import torch
@@ -227,7 +227,7 @@ def eikonal_step_fn(params, K, up):
     from cuda_knn_aggregate import knn_aggregate_aniso
 
     def step():
-        f_out, w_out, dists, idxs = knn_aggregate_aniso(*params, K)
+        f_out, w_out, dists, idxs = knn_aggregate_aniso(*params, K, tune=tune)
         dq = torch.autograd.grad(f_out, params[0], up(f_out), create_graph=True)[0]
         L = ((dq.norm(dim=-1) - 1.0) ** 2).sum()
         return torch.autograd.grad(L, params)
@@ -265,7 +265,7 @@ def _loss_grad(dq):
     return 2.0 * (n - 1.0) * dq / n
 
 
-def explicit_step_fn(x, K, v, cache):
+def explicit_step_fn(x, K, v, cache, tune=None):
     """The Eikonal step's gradients without autograd: forward, dq-only first order, the loss
     gradient h, and the second order of <h, dq> (the same launches the autograd step makes
     for these gradients), for capture."""
@@ -273,7 +273,7 @@ def explicit_step_fn(x, K, v, cache):
 
     def step():
         q, p, f, s, R = x
-        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, K, cache=cache)
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, s, R, K, cache=cache, tune=tune)
         up = v.expand_as(f_out)
         dq = _C.knn_aggregate_aniso_backward(up, None, q, p, f, s, R, None, idxs, cache=cache, check=False,
                                              needs=(True, False, False, False, False))[0]
@@ -530,15 +530,19 @@ def main():
 
     import torch
 
-    from cuda_knn_aggregate import _C, _lib
+    from cuda_knn_aggregate import _C, _lib, autotune
 
     dev = torch.device("cuda:0")
     b = make_batch(args.workload).to(dev)
     B, Q, P, F, K = (b.shape[k] for k in ("B", "Q", "P", "F", "K"))
+    # the search launch the autotuner picks for this shape class (timed on first use,
+    # outside the timed steps; the choice is kept in its cache file)
+    tune = autotune.tuned(b.q, b.p, b.sigma, b.R, K)
+    print(f"[bench] search tune {tune}", flush=True)
     params = [b.q, b.p, b.f, b.sigma, b.R]
     for t in params:
         t.requires_grad_(True)
-    step = eikonal_step_fn(params, K, lambda f_out: b.v.expand_as(f_out))
+    step = eikonal_step_fn(params, K, lambda f_out: b.v.expand_as(f_out), tune)
 
     print(f"[bench] workload {WORKLOADS[args.workload]}: warmup {args.warmup}", flush=True)
     for _ in range(args.warmup):
@@ -563,7 +567,7 @@ def main():
     if "knn_tiles" in kernels:
         st = torch.zeros(5, dtype=torch.int64, device=dev)
         _C.knn_aniso(b.q.detach(), b.p.detach(), b.sigma.detach(), b.R.detach(), K,
-                     tune={"search": "tiles", "stats": st})
+                     tune=dict(tune, stats=st))
         c = st.tolist()
         pairs["knn_tiles"] = c[3] * 64
         search_stats = {"waves": c[4], "super_tests_per_wave": c[0] / c[4], "tile_tests_per_wave": c[1] / c[4],
@@ -634,7 +638,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "B": B, "Q": Q, "P": P, "K": K, "F": F,
                    "step": "fwd + 1st-order (create_graph, upstream v.expand_as(f_out)) + 2nd-order Eikonal "
-                           "backward"},
+                           "backward", "search_tune": tune},
         "roofline": roof(dom),
         "roofline_gather": roof_agg,
         "kernels": kernels,
@@ -645,14 +649,14 @@ def main():
         print("[bench] captured step", flush=True)
         xs = [t.detach() for t in params]
         cache = _C.new_cache(B, Q, P, xs[0].dtype, dev)
-        tgraph = captured_step_ms(explicit_step_fn(xs, K, b.v, cache), args.steps)
+        tgraph = captured_step_ms(explicit_step_fn(xs, K, b.v, cache, tune), args.steps)
         out["captured_step"] = {"ms_per_step": tgraph, "value": B * Q / (tgraph * 1e-3), "unit": "queries/s",
                                 "step": "one HIP graph: forward, dq (v broadcast), loss gradient, second order"}
         del cache
         # the same step with a general per-query upstream (materialised N(0,1) rows)
         print("[bench] general-upstream step", flush=True)
         g_up = torch.randn(B, Q, F, generator=torch.Generator().manual_seed(7)).to(dev)
-        gstep = eikonal_step_fn(params, K, lambda f_out: g_up)
+        gstep = eikonal_step_fn(params, K, lambda f_out: g_up, tune)
         for _ in range(2):
             gstep()
         torch.cuda.synchronize()

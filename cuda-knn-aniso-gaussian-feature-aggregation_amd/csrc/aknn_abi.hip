@@ -223,6 +223,8 @@ void apply_geometry(aknn::PrunedLaunch& L, const aknn_dims* d, const aknn_tune* 
   if (variants && (t->sub_records == 4 || t->sub_records == 8 || t->sub_records == 16)) L.sub_rec = t->sub_records;
   if (variants && L.sub_rec == aknn::kSubRecDefault && (t->waves_per_eu == 4 || t->waves_per_eu == 8))
     L.waves = t->waves_per_eu;
+  if (t->mu_classes == 1 || t->mu_classes == 2 || t->mu_classes == 4 || t->mu_classes == 8)
+    L.mu_classes = t->mu_classes;
 }
 
 aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
@@ -248,6 +250,7 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   }
   L.sub_rec = aknn::kSubRecDefault;
   L.waves = 0;
+  L.mu_classes = aknn::kMuClassesDefault;
   L.nsuper_alloc = super_count(d, aknn::kSuperMin);
   set_super(L, d, aknn::kSuper);
   // Few query tiles (small or scattered query sets) cannot fill the device and the

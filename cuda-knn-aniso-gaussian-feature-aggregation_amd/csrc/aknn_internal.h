@@ -29,6 +29,7 @@ constexpr int kTile = 64;
 constexpr int kSuper = 16;
 constexpr int kSuperMin = 8;
 constexpr int kSubRecDefault = 8;
+constexpr int kMuClassesDefault = 1;  // record classes by bound within a tile
 constexpr int kSubRecMin = 4;
 constexpr int kQueryTile = 64;
 
@@ -55,6 +56,7 @@ struct PrunedLaunch {
   int32_t super_tiles;    // tiles per super-tile (8, 16 or 32)
   int32_t sub_rec;        // records per sub-tile of the tiled search (4, 8 or 16)
   int32_t waves;          // tiled search occupancy target (waves per SIMD; 0 = the list's default)
+  int32_t mu_classes;     // record classes by bound within a tile (1 = Morton order)
   int64_t nsuper_alloc;   // super-tiles at the smallest super-tile size (sizes the headers)
 };
 

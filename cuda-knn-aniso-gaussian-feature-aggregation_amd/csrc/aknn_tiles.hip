@@ -190,12 +190,20 @@ __global__ void codes_kernel(const T* __restrict__ xyz, Clouds cl, int64_t rows,
 // {lo x y z, hi x y z, mu (float, rounded down; < 0 = always scan), first code (bits)};
 // sub: [tiles][kSubsMax][8], of which the first kTile / SUBREC hold the same box and mu
 // (last value unused) of each SUBREC-record sub-tile.
+//
+// Record order within a tile (mu_classes = m): the tile's points are ranked by their
+// bound mu ("always scan" points first) and cut into m classes of equal size; records
+// are stored class by class, in Morton order within a class.  m = 1 keeps the Morton
+// order (sub-tiles are the tightest boxes); larger m groups the points that a weak
+// bound lets far queries reach (small mu: thin Gaussians) into their own sub-tiles, so
+// the others are skipped from further away.  The set of records, and hence every
+// result, does not depend on the order.
 template <typename T, int KEY, int SUBREC>
 __global__ __launch_bounds__(64) void tile_build_kernel(
     const T* __restrict__ p, const T* __restrict__ sigma, const T* __restrict__ R, Clouds cl,
     const int64_t* __restrict__ tile_prefix, const uint32_t* __restrict__ perm,
     const uint64_t* __restrict__ sorted_keys, T* __restrict__ rec, float* __restrict__ hdr,
-    float* __restrict__ sub, int32_t* __restrict__ tile_cnt) {
+    float* __restrict__ sub, int32_t* __restrict__ tile_cnt, int mu_classes) {
   const int64_t t = blockIdx.x;
   const int lane = threadIdx.x;
   int64_t b = 0;
@@ -218,10 +226,9 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
   float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
   double mu = __builtin_inf();
   bool bad = false;
-  T* out = rec + (t * kTile + lane) * 16;
+  T v[16];
   if (lane < cnt) {
     const int64_t row = perm[i0 + lane];
-    T v[16];
     for (int c = 0; c < 3; ++c) v[c] = p[row * 3 + c];
     // R / sigma may be absent for the Euclidean key, which never reads them
     for (int c = 0; c < 9; ++c) v[3 + c] = R ? R[row * 9 + c] : T(0);
@@ -229,7 +236,6 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
     const int idx = static_cast<int>(row - pb);
     if constexpr (sizeof(T) == 4) v[15] = __int_as_float(idx);
     else v[15] = __longlong_as_double(static_cast<long long>(idx));
-    for (int c = 0; c < 16; ++c) out[c] = v[c];
     for (int c = 0; c < 3; ++c) {
       // box in float, widened outward so it contains the exact coordinate
       const float f = static_cast<float>(v[c]);
@@ -242,9 +248,41 @@ __global__ __launch_bounds__(64) void tile_build_kernel(
   } else {
     // padding record: NaN position, so every key evaluated against it is NaN and it
     // never enters a top-K list (the scan reads whole sub-tiles)
-    for (int c = 0; c < 16; ++c) out[c] = c < 3 ? static_cast<T>(__builtin_nan("")) : T(0);
-    if constexpr (sizeof(T) == 4) out[15] = __int_as_float(kTileSentinel);
-    else out[15] = __longlong_as_double(static_cast<long long>(kTileSentinel));
+    for (int c = 0; c < 16; ++c) v[c] = c < 3 ? static_cast<T>(__builtin_nan("")) : T(0);
+    if constexpr (sizeof(T) == 4) v[15] = __int_as_float(kTileSentinel);
+    else v[15] = __longlong_as_double(static_cast<long long>(kTileSentinel));
+  }
+  // slot of this lane's record in the tile (a permutation of the lanes; padding last)
+  int slot = lane;
+  const int classes = mu_classes;
+  const double mkey = lane < cnt ? (bad ? -1.0 : mu) : __builtin_inf();
+  if (classes > 1) {
+    int r = 0;  // rank by (mu, lane)
+    for (int j = 0; j < 64; ++j) {
+      const double kj = __shfl(mkey, j, 64);
+      r += (kj < mkey) | ((kj == mkey) & (j < lane));
+    }
+    const int cls = lane < cnt ? r * classes / cnt : classes;
+    slot = 0;  // rank by (class, lane)
+    for (int j = 0; j < 64; ++j) {
+      const int cj = __shfl(cls, j, 64);
+      slot += (cj < cls) | ((cj == cls) & (j < lane));
+    }
+  }
+  T* out = rec + (t * kTile + slot) * 16;
+  for (int c = 0; c < 16; ++c) out[c] = v[c];
+  if (classes > 1) {
+    // the box values move to lane `slot` (a forward permute: every lane receives one)
+    auto push = [&](int x) { return __builtin_amdgcn_ds_permute(slot << 2, x); };
+    for (int c = 0; c < 3; ++c) {
+      lo3[c] = __int_as_float(push(__float_as_int(lo3[c])));
+      hi3[c] = __int_as_float(push(__float_as_int(hi3[c])));
+    }
+    const long long mb = __double_as_longlong(mu);
+    const long long lo_ = static_cast<uint32_t>(push(static_cast<int>(mb & 0xffffffffll)));
+    const long long hi_ = static_cast<long long>(push(static_cast<int>(mb >> 32))) << 32;
+    mu = __longlong_as_double(hi_ | lo_);
+    bad = push(static_cast<int>(bad)) != 0;
   }
   // Reductions over groups of SUBREC lanes (the sub-tiles), then over the wave (the tile).
   // min / max are exact, so the grouping does not change any value.
@@ -497,8 +535,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     sb = static_cast<int>(b * stiles[1]);
     se = sb + static_cast<int>(stiles[1]);
   }
-  // seed: last tile whose first code <= the wave's first query code (scalar search)
-  const uint32_t code0 = static_cast<uint32_t>(qkeys[q0] & 0xffffffffu);
+  // seed: last tile whose first code <= the code of the wave's middle query (scalar
+  // search): the walk then starts nearer to most lanes' own neighbourhoods
+  const uint32_t code0 = static_cast<uint32_t>(qkeys[min(q0 + 32, qend - 1)] & 0xffffffffu);
   int lo = tb, hi = te - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1353,13 +1392,13 @@ static hipError_t prep_points(const PrunedLaunch& L, const PrunedState& S, hipSt
   const T* rt = static_cast<const T*>(L.R);
   if (L.sub_rec == 4)
     hipLaunchKernelGGL((tile_build_kernel<T, KEY, 4>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
-                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt, L.mu_classes);
   else if (L.sub_rec == 16)
     hipLaunchKernelGGL((tile_build_kernel<T, KEY, 16>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
-                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt, L.mu_classes);
   else
     hipLaunchKernelGGL((tile_build_kernel<T, KEY, 8>), tg, dim3(64), 0, st, p, sg, rt, L.cl, S.ptp, S.pperm,
-                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt);
+                       S.pkeys_s, static_cast<T*>(S.rec), S.hdr, S.sub, S.tcnt, L.mu_classes);
   hipLaunchKernelGGL(super_build_kernel, dim3(static_cast<unsigned>((L.nsuper + bs - 1) / bs)), dim3(bs), 0, st,
                      L.cl, S.ptp, S.stp, L.nsuper, S.hdr, S.shdr, static_cast<int>(L.super_tiles));
   return hipGetLastError();

@@ -36,6 +36,7 @@ def knn_aggregate_aniso(
     K: int,
     *,
     key: str = "aniso",
+    tune=None,
 ):
     """k-NN feature aggregate using anisotropic gaussian weights.
 
@@ -55,7 +56,13 @@ def knn_aggregate_aniso(
         dists: [B, Q, K] distances. Not differentiable.
         idxs: [B, Q, K] k-NN indices (int64). Not differentiable.
     """
-    return KNNAnisotropicAggregate.apply(q, p, f, sigma, R, K, key)
+    if isinstance(tune, str):
+        if tune != "auto":
+            raise ValueError("tune must be None, 'auto' or a dict of search tunables")
+        from . import autotune  # noqa: PLC0415
+
+        tune = autotune.tuned(q.detach(), p.detach(), sigma.detach(), R.detach(), K, key=key)
+    return KNNAnisotropicAggregate.apply(q, p, f, sigma, R, K, key, tune)
 
 
 def knn_aggregate_aniso_lookup(
@@ -101,7 +108,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
     """Neighbour search + aggregation (reference: cuda_knn_aggregate/__init__.py:64-117)."""
 
     @staticmethod
-    def forward(ctx, q, p, f, sigma, R, K, key="aniso"):
+    def forward(ctx, q, p, f, sigma, R, K, key="aniso", tune=None):
         # The cache (spatial row orders, packed point records) only speeds up the
         # backward kernels; it is filled when a backward can follow.
         # (none without query rows: the forward then writes nothing into it)
@@ -109,7 +116,8 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         if (any(ctx.needs_input_grad[:5]) and q.is_cuda and p.dim() == 3 and q.dim() == 3
                 and q.shape[0] * q.shape[1] > 0):
             cache = _C.new_cache(q.shape[0], q.shape[1], p.shape[1], q.dtype, q.device)
-        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key, cache=cache)
+        f_out, w_out, dists, idxs = _C.knn_aggregate_aniso_forward(q, p, f, sigma, R, K, key=key, tune=tune,
+                                                                   cache=cache)
         # Like the reference (__init__.py:89); the backward kernels recompute the
         # weights from idxs and never read w_out.
         ctx.save_for_backward(q, p, f, sigma, R, w_out, idxs, *(cache or ()))
@@ -125,7 +133,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         # Only grad_f_out enters (grad_w_out and grad_dists are ignored, as in the
         # reference, csrc/backward.cu:93-94): no grad on f_out means no gradient at all.
         if grad_f_out is None:
-            return (None,) * 7
+            return (None,) * 8
         if grad_w_out is None:  # never read: a broadcast zero instead of a Q x K fill
             grad_w_out = w_out.new_zeros(()).expand_as(w_out)
         needs = tuple(ctx.needs_input_grad[:5])
@@ -147,7 +155,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
             needs,
             cache,
         )
-        return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None
+        return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None, None
 
 
 class KNNLookupAnisotropicAggregate(KNNAnisotropicAggregate):

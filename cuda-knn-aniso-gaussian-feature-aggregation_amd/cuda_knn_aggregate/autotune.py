@@ -6,8 +6,9 @@ differs):
 * the algorithm and its queries per wave: the tiled search (64 queries per wave, one per
   lane), the per-query walk (one query per wave) or the brute-force scan (64 x 1, 2 or 4
   queries per wave, register lists per lane);
-* point-tile geometry: records per sub-tile the tiled search tests and scans (4, 8, 16)
-  and tiles per super-tile (8, 16, 32, also for the per-query walk);
+* point-tile geometry: records per sub-tile the tiled search tests and scans (4, 8, 16),
+  tiles per super-tile (8, 16, 32, also for the per-query walk) and the record order
+  within a tile (Morton order, or 2 or 4 classes by pruning bound);
 * occupancy: the tiled search's waves per SIMD (4, 6 by default, 8);
 * LDS layout of the brute scan: waves splitting a query tile's points (1..8), whose
   lists merge through LDS.
@@ -31,6 +32,8 @@ from . import _C
 
 CANDIDATES = (
     [{"search": "tiles", "sub_records": r, "super_tiles": n} for r in (8, 4, 16) for n in (16, 8, 32)]
+    + [{"search": "tiles", "sub_records": r, "super_tiles": n, "mu_classes": c}
+       for c in (2, 4) for r in (8, 4) for n in (16, 8)]
     + [{"search": "tiles", "waves_per_eu": w} for w in (4, 8)]
     + [{"search": "wave", "super_tiles": n} for n in (16, 8, 32)]
     + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]

@@ -276,6 +276,9 @@ class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
         saved = [grad_f_out, q, p, f, sigma, R, idxs] + (list(csr) if csr else []) + (list(cache) if cache else [])
         ctx.save_for_backward(*saved)
         ctx.layout, ctx.key, ctx.has_csr, ctx.has_cache = layout, key, csr is not None, cache is not None
+        # outputs the loss does not use arrive as None, not as zeros that would select the
+        # full second order
+        ctx.set_materialize_grads(False)
         return tuple(out)
 
     @staticmethod

@@ -128,6 +128,9 @@ typedef struct aknn_tune {
   int32_t super_tiles;      /* tiles and per-query walk: tiles per super-tile (8, 16, 32) */
   int32_t waves_per_eu;     /* tiles: occupancy target of the search (4 or 8 waves per
                                SIMD; FP32 with 9 to 16 neighbours and 8-record sub-tiles) */
+  int32_t mu_classes;       /* tiles and per-query walk: the records of a point tile are
+                               stored in 1, 2, 4 or 8 classes by their pruning bound
+                               (1 = Morton order), which shapes the sub-tiles       */
 } aknn_tune;
 
 /* Point index (no counterpart in the reference; for callers that keep clouds resident

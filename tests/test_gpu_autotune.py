@@ -54,3 +54,23 @@ def test_tuner_cache_round_trip(cuda, tmp_path, monkeypatch):
     ref = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16)
     got = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune=best)
     assert torch.equal(got[1], ref[1])
+
+
+def test_public_call_with_tuner(cuda, tmp_path, monkeypatch):
+    """``knn_aggregate_aniso(..., tune="auto")`` runs the tuner's choice: same outputs and
+    gradients as the default launch."""
+    from cuda_knn_aggregate import knn_aggregate_aniso
+
+    monkeypatch.setenv("AKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_mem", {})
+    b = workloads.clustered_batch(Q=20000, P=9000, K=16, F=8, seed=9).to(cuda)
+    res = []
+    for tune in (None, "auto"):
+        ins = [t.clone().requires_grad_(True) for t in (b.q, b.p, b.f, b.sigma, b.R)]
+        out = knn_aggregate_aniso(*ins, 16, tune=tune)
+        out[0].square().sum().backward()
+        res.append([o.detach() for o in out] + [x.grad for x in ins])
+    for x, y in zip(*res):
+        assert torch.equal(x, y)
+    with pytest.raises(ValueError):
+        knn_aggregate_aniso(b.q, b.p, b.f, b.sigma, b.R, 16, tune="fast")

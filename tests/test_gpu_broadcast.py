@@ -250,7 +250,16 @@ def test_ragged_full_second_order_matches_dense(cuda):
             assert torch.equal(x[a:e], y[0]), (c, n)
 
 
-def test_ragged_full_second_order_autograd(cuda):
+def _loss_all(dq, dp, df):
+    return dp.square().sum() + df.sin().sum() + dq.square().sum()
+
+
+def _loss_dp(dq, dp, df):
+    return dp.square().sum()  # no upstream on dq: grad_grad_q arrives as None
+
+
+@pytest.mark.parametrize("loss", [_loss_all, _loss_dp])
+def test_ragged_full_second_order_autograd(cuda, loss):
     """A loss on the ragged first-order dp and df reaches the second order through
     autograd (it used to be dropped) and matches the dense autograd per cloud."""
     r = workloads.ragged_small(seed=3)
@@ -260,7 +269,7 @@ def test_ragged_full_second_order_autograd(cuda):
     gen = torch.Generator().manual_seed(12)
     g = torch.randn(f_out.shape, generator=gen).to(cuda)
     dq, dp, df = torch.autograd.grad(f_out, ins[:3], g, create_graph=True)
-    (dp.square().sum() + df.sin().sum() + dq.square().sum()).backward()
+    loss(dq, dp, df).backward()
     got = [x.grad.clone() for x in ins]
     qo, po = r.q_off.tolist(), r.p_off.tolist()
     for c in range(layout.B):
@@ -271,7 +280,7 @@ def test_ragged_full_second_order_autograd(cuda):
                for t, (a, e) in zip(ins, [(qs, qe)] + [(ps, pe)] * 4)]
         fo = knn_aggregate_aniso(*cin, r.K)[0]
         cq, cp, cf = torch.autograd.grad(fo, cin[:3], g[qs:qe].unsqueeze(0), create_graph=True)
-        (cp.square().sum() + cf.sin().sum() + cq.square().sum()).backward()
+        loss(cq, cp, cf).backward()
         for n, x, (a, e), y in zip("q p f sigma R".split(), got, [(qs, qe)] + [(ps, pe)] * 4, cin):
             err = rel(x[a:e], y.grad[0])
             assert err <= 1e-6, (c, n, err)
