@@ -251,6 +251,10 @@ aknn::PrunedLaunch pruned_launch(const aknn_dims* d) {
   L.sub_rec = aknn::kSubRecDefault;
   L.waves = 0;
   L.mu_classes = aknn::kMuClassesDefault;
+  // Ragged batches: resident blocks over the (cloud, query tile) work list, measured
+  // faster than one block per query tile there (uneven clouds); dense batches keep the
+  // one-block-per-tile grid, measured faster on clustered clouds.
+  L.persistent = ragged(d) ? 1 : 0;
   L.nsuper_alloc = super_count(d, aknn::kSuperMin);
   set_super(L, d, aknn::kSuper);
   // Few query tiles (small or scattered query sets) cannot fill the device and the
@@ -321,6 +325,7 @@ int run_knn(const aknn_dims* d, const aknn_tune* tune, const void* q, const void
     PL.index = tune ? tune->index : nullptr;
     PL.wave = pick_wave(d, tune) ? 1 : 0;
     apply_geometry(PL, d, tune);
+    if (tune && tune->persistent != 0) PL.persistent = tune->persistent > 0;
     if (tune && tune->point_split > 0) {
       int w = 1;
       while (2 * w <= tune->point_split && 2 * w <= PL.ways_max) w *= 2;

@@ -57,6 +57,7 @@ struct PrunedLaunch {
   int32_t sub_rec;        // records per sub-tile of the tiled search (4, 8 or 16)
   int32_t waves;          // tiled search occupancy target (waves per SIMD; 0 = the list's default)
   int32_t mu_classes;     // record classes by bound within a tile (1 = Morton order)
+  int32_t persistent;     // tiled search: resident blocks over a work list of query tiles
   int64_t nsuper_alloc;   // super-tiles at the smallest super-tile size (sizes the headers)
 };
 

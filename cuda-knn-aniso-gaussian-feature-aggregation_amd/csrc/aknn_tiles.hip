@@ -23,6 +23,10 @@
 // lmin comes from the closed-form 3x3 symmetric eigenvalues in FP64 minus a margin, the
 // result is scaled by (1 - 1e-6) to absorb the FP32 rounding of the tile test itself.
 // Points with any negative scale, or non-finite data, mark their tile "always scan".
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "aknn_bounds.h"
 #include "aknn_common.h"
 #include "aknn_internal.h"
@@ -48,6 +52,7 @@ struct PrunedState {
   void* lkey;       // split walk: per (sorted query, way) top-K keys   [qrows][ways][K]
   int32_t* lid;     //                                    and indices
   void* bnd;        // split walk: shared per-query bound (ordered encoding) [qrows]
+  unsigned* work;   // persistent search: next-item counters, one per XCD [kXcds]
   size_t total;
 };
 
@@ -479,16 +484,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     const float* __restrict__ sub, const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists,
     int64_t* __restrict__ idxs, unsigned long long* __restrict__ stats, int ways, T* __restrict__ lkey,
     int32_t* __restrict__ lid, typename OrdKey<T>::U* __restrict__ bnd, const T* __restrict__ pts,
-    const T* __restrict__ sig, const T* __restrict__ rot, int super) {
+    const T* __restrict__ sig, const T* __restrict__ rot, int super, unsigned* __restrict__ work,
+    int64_t nitems) {
   using Ord = OrdKey<T>;
   constexpr int kSubs = kTile / SUBREC;
   const int lane = threadIdx.x;
-  int64_t b, q0;
-  // XCD-aware order of the (spatially sorted) query tiles: each XCD's L2 then holds
-  // the point tiles of one region.  The `ways` blocks of one query tile are adjacent,
-  // so they share an XCD (and the bound they exchange stays in its L2).
+  // Work items: lin0 = (query tile, way) in XCD-aware order of the (spatially sorted)
+  // query tiles, so that each XCD's L2 holds the point tiles of one region; the `ways`
+  // items of one query tile are adjacent, so they share an XCD (and the bound they
+  // exchange stays in its L2).  One block per item, or (work != null) persistent
+  // blocks: the items are cut into kXcds ranges, one per XCD, and each block takes the
+  // next item of its own XCD's range (blocks are dealt to the XCDs round robin), then
+  // of the following ranges once that one is used up; every block leaves once all
+  // ranges are used up.  The counters start at zero (cleared before the launch).
   const int64_t nblk = static_cast<int64_t>(gridDim.x) * gridDim.y;
-  const int64_t lin0 = xcd_block(static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x, nblk);
+  const int64_t hw = static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+  int xr = static_cast<int>(hw % kXcds), tries = 0;
+  auto next_item = [&]() __attribute__((always_inline)) -> int64_t {
+    const int64_t qn = nitems / kXcds, rn = nitems % kXcds;
+    while (tries < kXcds) {
+      const int64_t lo = xr * qn + (xr < rn ? xr : rn);
+      const int64_t hi = lo + qn + (xr < rn ? 1 : 0);
+      unsigned got = 0;
+      if (lane == 0) got = atomicAdd(work + xr, 1u);
+      got = __builtin_amdgcn_readfirstlane(got);  // wave-uniform (scalar) from here on
+      if (lo + static_cast<int64_t>(got) < hi) return lo + static_cast<int64_t>(got);
+      xr = xr + 1 == kXcds ? 0 : xr + 1;
+      ++tries;
+    }
+    return -1;
+  };
+  auto walk = [&](const int64_t lin0) __attribute__((always_inline)) {
+  int64_t b, q0;
   const int64_t way = lin0 % ways;
   const int64_t lin = lin0 / ways;
   if (qtiles) {
@@ -501,7 +528,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     b = lo;
     q0 = cl.q_begin(b) + (t - qtiles[b]) * 64;
   } else {
-    const int64_t per = gridDim.x / ways;  // query tiles per cloud
+    const int64_t per = (cl.Q + 63) / 64;  // query tiles per cloud
     b = lin / per;
     q0 = cl.q_begin(b) + (lin % per) * 64;
   }
@@ -680,6 +707,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     atomicAdd(stats + 3, static_cast<unsigned long long>(n_pts));
     atomicAdd(stats + 4, 1ull);
   }
+  };
+  for (int64_t item = work ? next_item() : xcd_block(hw, nblk); item >= 0; item = work ? next_item() : -1)
+    walk(item);
 }
 
 // Super-tile headers: box union and min mu (negative if any tile is "always scan")
@@ -1186,15 +1216,51 @@ static hipError_t launch_wave(const PrunedLaunch& L, const PrunedState& S, hipSt
   return hipGetLastError();
 }
 
+// Blocks of `threads` threads of kernel `fn` resident on the current device at once (0 if
+// unknown), cached per kernel and device.
+static int64_t resident_blocks(const void* fn, int threads) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int64_t> cache;
+  std::lock_guard<std::mutex> guard(mu);
+  const auto it = cache.find({fn, dev});
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const int64_t r = static_cast<int64_t>(per_cu) * cus;
+  cache[{fn, dev}] = r;
+  return r;
+}
+
+__global__ void zero_u32_kernel(unsigned* __restrict__ p, int n) {
+  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0u;
+}
+
 template <typename T, int KMAX, int KEY, int SUBREC, int WAVES>
 static void launch_search(const PrunedLaunch& L, const PrunedState& S, dim3 grid, int ways, hipStream_t st) {
   using U = typename OrdKey<T>::U;
-  hipLaunchKernelGGL((knn_pruned_kernel<T, KMAX, KEY, SUBREC, WAVES>), grid, dim3(64), 0, st,
+  auto kern = knn_pruned_kernel<T, KMAX, KEY, SUBREC, WAVES>;
+  const int64_t nitems = static_cast<int64_t>(grid.x) * grid.y;
+  unsigned* work = nullptr;
+  if (L.persistent) {  // as many blocks as fit on the device at once, over the work items
+    const int64_t res = resident_blocks(reinterpret_cast<const void*>(kern), 64);
+    if (res > 0 && res < nitems) {
+      grid = dim3(static_cast<unsigned>(res));
+      work = S.work;
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, work, kXcds);
+    }
+  }
+  hipLaunchKernelGGL(kern, grid, dim3(64), 0, st,
                      static_cast<const T*>(L.q), L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp,
                      S.stp, static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
                      static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
                      static_cast<U*>(S.bnd), static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
-                     static_cast<const T*>(L.R), static_cast<int>(L.super_tiles));
+                     static_cast<const T*>(L.R), static_cast<int>(L.super_tiles), work, nitems);
 }
 
 template <typename T, int KMAX, int KEY>
@@ -1298,6 +1364,7 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
   S.qkeys_s = reinterpret_cast<uint64_t*>(c.take(L.qrows * 8));
   S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
   S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
+  S.work = reinterpret_cast<unsigned*>(c.take(kXcds * 4));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
   if (L.ways > 1) {

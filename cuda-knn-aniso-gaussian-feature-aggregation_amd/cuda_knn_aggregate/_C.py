@@ -127,6 +127,7 @@ def _tune(tune) -> Optional[_lib.Tune]:
     t.super_tiles = int(tune.get("super_tiles", 0))
     t.waves_per_eu = int(tune.get("waves_per_eu", 0))
     t.mu_classes = int(tune.get("mu_classes", 0))
+    t.persistent = int(tune.get("persistent", 0))
     st = tune.get("stats")  # optional int64 device tensor of 5 counters
     t.stats = st.data_ptr() if st is not None else None
     ix = tune.get("index")  # optional point index from build_index (same points)

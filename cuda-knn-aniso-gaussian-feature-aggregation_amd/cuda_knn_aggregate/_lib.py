@@ -107,7 +107,7 @@ class Tune(ctypes.Structure):
     _fields_ = [("queries_per_lane", ctypes.c_int32), ("point_split", ctypes.c_int32),
                 ("search", ctypes.c_int32), ("sub_records", ctypes.c_int32), ("stats", ctypes.c_void_p),
                 ("index", ctypes.c_void_p), ("super_tiles", ctypes.c_int32), ("waves_per_eu", ctypes.c_int32),
-                ("mu_classes", ctypes.c_int32)]
+                ("mu_classes", ctypes.c_int32), ("persistent", ctypes.c_int32)]
 
 
 SEARCH = {"auto": 0, "brute": 1, "tiles": 2, "wave": 3}

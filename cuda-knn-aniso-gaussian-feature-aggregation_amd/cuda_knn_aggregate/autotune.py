@@ -9,7 +9,9 @@ differs):
 * point-tile geometry: records per sub-tile the tiled search tests and scans (4, 8, 16),
   tiles per super-tile (8, 16, 32, also for the per-query walk) and the record order
   within a tile (Morton order, or 2 or 4 classes by pruning bound);
-* occupancy: the tiled search's waves per SIMD (4, 6 by default, 8);
+* occupancy: the tiled search's waves per SIMD (4, 6 by default, 8), and its launch: one
+  block per query tile, or as many blocks as fit on the device taking query tiles from a
+  work list;
 * LDS layout of the brute scan: waves splitting a query tile's points (1..8), whose
   lists merge through LDS.
 
@@ -35,6 +37,7 @@ CANDIDATES = (
     + [{"search": "tiles", "sub_records": r, "super_tiles": n, "mu_classes": c}
        for c in (2, 4) for r in (8, 4) for n in (16, 8)]
     + [{"search": "tiles", "waves_per_eu": w} for w in (4, 8)]
+    + [{"search": "tiles", "persistent": s, "super_tiles": n} for s in (1, -1) for n in (16, 8)]
     + [{"search": "wave", "super_tiles": n} for n in (16, 8, 32)]
     + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]
 )

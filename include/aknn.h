@@ -131,6 +131,10 @@ typedef struct aknn_tune {
   int32_t mu_classes;       /* tiles and per-query walk: the records of a point tile are
                                stored in 1, 2, 4 or 8 classes by their pruning bound
                                (1 = Morton order), which shapes the sub-tiles       */
+  int32_t persistent;       /* tiles: 1 = as many blocks as fit on the device at once,
+                               taking (cloud, query tile) work items from per-XCD
+                               counters; -1 = one block per work item; 0 = default
+                               (1 for ragged batches, -1 for dense ones)           */
 } aknn_tune;
 
 /* Point index (no counterpart in the reference; for callers that keep clouds resident

@@ -74,3 +74,66 @@ def test_public_call_with_tuner(cuda, tmp_path, monkeypatch):
         assert torch.equal(x, y)
     with pytest.raises(ValueError):
         knn_aggregate_aniso(b.q, b.p, b.f, b.sigma, b.R, 16, tune="fast")
+
+
+def _same(a, b):
+    return all(torch.equal(x, y) for x, y in zip(a, b))
+
+
+def test_persistent_search_same_bits(cuda):
+    """Resident blocks over the work list give the one-block-per-tile launch's neighbours,
+    with more work items than resident blocks: dense, split walk (8 ways), ragged."""
+    b = workloads.clustered_batch(Q=600000, P=30000, K=16, F=4, seed=21).to(cuda)
+    grid = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune={"search": "tiles", "persistent": -1})
+    pers = _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune={"search": "tiles", "persistent": 1})
+    assert _same(grid, pers)
+    b = workloads.clustered_batch(Q=65536, P=65536, K=16, F=4, seed=22).to(cuda)
+    for ways in (8, 4):
+        tg = {"search": "tiles", "persistent": -1, "point_split": ways}
+        tp = dict(tg, persistent=1)
+        assert _same(_C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune=tg),
+                     _C.knn_aniso(b.q, b.p, b.sigma, b.R, 16, tune=tp)), ways
+    r = workloads.ragged_batch(seed=0)
+    layout = rg.RaggedLayout(r.q_off, r.p_off, cuda)
+    d = r.to(cuda)
+    f1 = d.f[:, :4].contiguous()
+    outs = [rg.forward(layout, d.q, d.p, f1, d.sigma, d.R, r.K, tune={"search": "tiles", "persistent": s})
+            for s in (-1, 1, 0)]
+    assert _same(outs[0], outs[1]) and _same(outs[0], outs[2])
+
+
+def test_persistent_search_captured(cuda):
+    """The persistent launch (its counters cleared by a kernel, not a memset) replays from a
+    HIP graph with the eager bits, also after new query values."""
+    r = workloads.ragged_batch(seed=1)
+    layout = rg.RaggedLayout(r.q_off, r.p_off, cuda)
+    d = r.to(cuda)
+    q = d.q.clone()
+    f1 = d.f[:, :4].contiguous()
+    tune = {"search": "tiles", "persistent": 1}
+
+    def fwd():
+        return rg.forward(layout, q, d.p, f1, d.sigma, d.R, r.K, tune=tune)
+
+    eager = [t.clone() for t in fwd()]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            fwd()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        static = fwd()
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        assert _same(static, eager)
+        torch.randn(1 << 20, device=cuda).sum().item()
+    with torch.no_grad():
+        q.add_(0.001 * torch.randn(q.shape, generator=torch.Generator().manual_seed(3)).to(cuda))
+    fresh = [t.clone() for t in fwd()]
+    graph.replay()
+    torch.cuda.synchronize()
+    assert _same(static, fresh) and not torch.equal(fresh[3], eager[3])
