@@ -89,6 +89,27 @@ __device__ __forceinline__ int64_t xcd_block(int64_t hw, int64_t n) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Exact unsigned 32-bit division by a divisor fixed for the whole kernel (the neighbour
+// count K of an entry number q*K + k), by multiply-high and shifts: with l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1 and t = mulhi(m, n), n / d = (t + ((n - t) >> min(l,
+// 1))) >> max(l - 1, 0) for every 32-bit n (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", 1994, section 4).
+struct FastDiv {
+  uint32_t m;
+  int s1, s2;
+  __device__ __forceinline__ explicit FastDiv(uint32_t d) {
+    int l = 0;
+    while ((1u << l) < d) ++l;
+    m = static_cast<uint32_t>(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    s1 = l < 1 ? l : 1;
+    s2 = l > 1 ? l - 1 : 0;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
 // ------------------------------------------------------------------ key
 // The neighbour key, one rounding per operation in the reference's association
 // order (csrc/forward.cu:85-92 and :154).  Contraction into FMA is disabled here so

@@ -50,6 +50,20 @@ __device__ __forceinline__ void pstatic_for(Fn&& fn) {
   }
 }
 
+// With one point row per wave (G = 4) the row index is the same in every lane: said so
+// to the compiler, the row's CSR bounds, point data and cloud lookup become scalar
+// loads and the entry loop a scalar loop.
+template <bool UNIFORM>
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  if constexpr (UNIFORM) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+  } else {
+    return v;
+  }
+}
+
 // Sum over the G 16-lane groups of a wave that share one point row (fixed order:
 // (g0 + g1) + (g2 + g3); every group ends with the same bits).
 template <int G, typename T>
@@ -97,11 +111,12 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int32_t* __restrict__ ent = L.csr_ent;
   const int64_t F = L.F;
   const int K = static_cast<int>(L.K);
+  const FastDiv kdiv(static_cast<uint32_t>(K));  // entry -> query row (entries < 2^31)
 
   const int r = threadIdx.x & 15;
   const int gq = (threadIdx.x >> 4) % G;
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t gi = blk * (16 / G) + (threadIdx.x >> 4) / G;
+  const int64_t gi = wave_uniform<G == 4>(blk * (16 / G) + (threadIdx.x >> 4) / G);
   const bool row_ok = gi < L.rows;
   const int64_t gc = row_ok ? gi : L.rows - 1;
   // spatial processing order (when given): neighbouring points share queries, so the
@@ -143,7 +158,7 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     const int e = eb + 16 * gq + r;
     const bool have = e < e1;
     const int entry = have ? ent[e] : 0;
-    const int64_t qrow = have ? static_cast<int64_t>(entry / K) : 0;
+    const int64_t qrow = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
     const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
@@ -305,11 +320,12 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
   const int32_t* __restrict__ ent = L.csr_ent;
   const int64_t F = L.F;
   const int K = static_cast<int>(L.K);
+  const FastDiv kdiv(static_cast<uint32_t>(K));  // entry -> query row (entries < 2^31)
 
   const int r = threadIdx.x & 15;
   const int gq = (threadIdx.x >> 4) % G;
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t gi = blk * (16 / G) + (threadIdx.x >> 4) / G;
+  const int64_t gi = wave_uniform<G == 4>(blk * (16 / G) + (threadIdx.x >> 4) / G);
   const bool row_ok = gi < L.rows;
   const int64_t gc = row_ok ? gi : L.rows - 1;
   // spatial processing order (when given): neighbouring points share queries, so the
@@ -367,7 +383,7 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
     const int e = eb + 16 * gq + r;
     const bool have = e < e1;
     const int entry = have ? ent[e] : 0;
-    const int64_t qrow = have ? static_cast<int64_t>(entry / K) : 0;
+    const int64_t qrow = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
     const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
@@ -803,7 +819,7 @@ __global__ void csr_keys4_kernel(const int64_t* __restrict__ idxs, Clouds cl, ui
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t e = 4 * t;
   if (e >= entries) return;
-  const int64_t qrow = e / K;
+  const int64_t qrow = FastDiv(K).div(e);
   const uint32_t pb = static_cast<uint32_t>(cl.p_begin(cl.cloud_of_query(qrow)));
   const longlong2 a = *reinterpret_cast<const longlong2*>(idxs + e);
   const longlong2 c = *reinterpret_cast<const longlong2*>(idxs + e + 2);
