@@ -33,6 +33,45 @@ AKNN_HD inline float next_down(float f) {
 }
 AKNN_HD inline float next_up(float f) { return -next_down(-f); }
 
+// A lower bound of the smallest eigenvalue of the symmetric 3x3 matrix M (trace S >=
+// ||M||_F), by cyclic Jacobi rotations in FP64: each rotation zeroes one off-diagonal
+// pair (tan t of the rotation angle the root of smaller magnitude of t^2 + 2 cot(2 angle)
+// t - 1 = 0), so the iterate converges to diag(eigenvalues).  By Weyl's inequality the
+// eigenvalues of the final iterate lie within its off-diagonal Frobenius norm of its
+// diagonal, and the rotations in floating point are backward stable (the final iterate
+// is an exact similarity of M plus a perturbation of a few dozen ulp of ||M||, covered
+// by the 1e-12 S margin).  Unlike the closed form through acos, whose error grows to
+// sqrt(eps) of the spread when two eigenvalues nearly coincide, this stays within
+// eps-level of ||M||.
+AKNN_HD inline double lambda_min_lower(const double (&M)[3][3], double S) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = M[i][j];
+  for (int sweep = 0; sweep < 10; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    if (!(off > 1e-40 * S * S)) break;
+    for (int pair = 0; pair < 3; ++pair) {
+      const int p = pair == 2 ? 1 : 0, q = pair == 0 ? 1 : 2, r = 3 - p - q;
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+      const double at = fabs(theta);
+      // t = sign(theta) / (|theta| + sqrt(theta^2 + 1)), ~1/(2 theta) for huge theta
+      const double t = (at > 1e150 ? 0.5 / at : 1.0 / (at + sqrt(at * at + 1.0))) * (theta < 0.0 ? -1.0 : 1.0);
+      const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+      const double arp = a[r][p], arq = a[r][q];
+      a[p][p] -= t * apq;
+      a[q][q] += t * apq;
+      a[p][q] = a[q][p] = 0.0;
+      a[r][p] = a[p][r] = c * arp - sn * arq;
+      a[r][q] = a[q][r] = sn * arp + c * arq;
+    }
+  }
+  const double off = sqrt(2.0 * (a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2]));
+  const double d = fmin(fmin(a[0][0], a[1][1]), a[2][2]);
+  return d - off - 1e-12 * S;
+}
+
 // Certified lower bound mu_n of key(q, n) / |q - p_n|^2 for a point with rotation r9
 // (row-major) and scales s3, for keys evaluated in T; negative means "no bound" (the
 // box must always be scanned): negative or non-finite scales, non-finite rotations.
@@ -49,32 +88,13 @@ AKNN_HD inline double point_mu(const T* r9, const T* s3) {
     if (!__builtin_isfinite(R[i])) return -1.0;
   for (int i = 0; i < 3; ++i)
     if (!(s[i] >= 0.0) || !__builtin_isfinite(s[i])) return -1.0;
-  // M = R^T diag(s) R (symmetric 3x3)
+  // M = R^T diag(s) R (symmetric positive semidefinite 3x3)
   double M[3][3];
   for (int a = 0; a < 3; ++a)
     for (int c = 0; c < 3; ++c)
       M[a][c] = s[0] * R[a] * R[c] + s[1] * R[3 + a] * R[3 + c] + s[2] * R[6 + a] * R[6 + c];
-  const double S = M[0][0] + M[1][1] + M[2][2];
-  const double m = S / 3.0;
-  const double p1 = M[0][1] * M[0][1] + M[0][2] * M[0][2] + M[1][2] * M[1][2];
-  const double a0 = M[0][0] - m, a1 = M[1][1] - m, a2 = M[2][2] - m;
-  const double p2 = a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * p1;
-  double lmin;
-  if (p2 <= 0.0) {
-    lmin = m;
-  } else {
-    const double pp = sqrt(p2 / 6.0);
-    const double b00 = a0 / pp, b11 = a1 / pp, b22 = a2 / pp;
-    const double b01 = M[0][1] / pp, b02 = M[0][2] / pp, b12 = M[1][2] / pp;
-    const double det = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
-                       b02 * (b01 * b12 - b11 * b02);
-    double rr = det / 2.0;
-    rr = rr < -1.0 ? -1.0 : (rr > 1.0 ? 1.0 : rr);
-    const double phi = acos(rr) / 3.0;
-    lmin = m + 2.0 * pp * cos(phi + 2.0943951023931957);  // smallest eigenvalue
-  }
-  // closed-form eigenvalues are accurate to a few ulp of the largest one
-  lmin -= 1e-12 * (fabs(m) + S);
+  const double S = M[0][0] + M[1][1] + M[2][2];  // trace >= ||M||_F for PSD M
+  const double lmin = lambda_min_lower(M, S);
   if (!(lmin > 0.0)) return 0.0;
   const double mu = (1.0 - g) * (lmin - 2.0 * g * sqrt(lmin * S));
   return mu > 0.0 ? mu : 0.0;

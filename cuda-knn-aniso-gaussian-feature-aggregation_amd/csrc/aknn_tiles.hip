@@ -20,7 +20,7 @@
 // for the scan's FP32 evaluation (one rounding per op, unit roundoff u, g = 4u/(1-4u)):
 //   key >= (1 - g) (D - 2 g |d| sqrt(D S)),   S = sum_i s_i |R_i|^2 = trace(M)
 // which is increasing in D where it matters, hence key >= (1-g)(lmin - 2 g sqrt(lmin S)) |d|^2.
-// lmin comes from the closed-form 3x3 symmetric eigenvalues in FP64 minus a margin, the
+// lmin comes from Jacobi rotations of the 3x3 matrix in FP64 minus a margin, the
 // result is scaled by (1 - 1e-6) to absorb the FP32 rounding of the tile test itself.
 // Points with any negative scale, or non-finite data, mark their tile "always scan".
 #include <map>
