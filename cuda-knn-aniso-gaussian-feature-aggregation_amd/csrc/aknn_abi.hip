@@ -904,10 +904,13 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
                            const void* R, const int64_t* idxs, const int32_t* csr_off,
                            const int32_t* csr_ent, const aknn_cache* cache_in, void* ws,
                            size_t ws_bytes, void* dgf, void* dq, void* dp, void* df, void* dsigma,
-                           void* dR, void* stream) {
+                           void* dR, void* stream, const void* gw = nullptr, void* dgw = nullptr) {
   int rc = check_dims(d, true);
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool need_dgw = second && (needs & AKNN_NEED_DGRAD_W_OUT) && query_rows(d) > 0;
+  needs &= ~AKNN_NEED_DGRAD_W_OUT;
+  if (need_dgw && !dgw) return fail(AKNN_E_INVALID, "d_grad_w_out requested but NULL");
   const bool full = second && up.any();
   if (full && !(needs & AKNN_SECOND_FULL))
     return fail(AKNN_E_INVALID, "grad_grad_p / _f / _sigma / _R given: set AKNN_SECOND_FULL in needs");
@@ -971,14 +974,17 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.order = cache ? cache->query_order : nullptr;
     L.rec = cache ? cache->point_records : nullptr;
     L.dots = dots;
+    L.gw = gw;
+    L.dgw = need_dgw ? dgw : nullptr;
     {
       Span sp(second ? "dq_dots2" : "dq_dots", st);
       e = aknn::launch_dq_dots(L, second, st);
     }
     if (e != hipSuccess) return hip_fail(e, "dq_dots");
   }
+  const bool dgw_by_dots = dq_dots && second;  // the dot-record kernel wrote d grad_w_out
   const bool want_q = ((needs & AKNN_NEED_DQ) && !via_contrib && !dq_dots) ||
-                      (second && (needs & AKNN_NEED_DGRAD_F_OUT));
+                      (second && (needs & AKNN_NEED_DGRAD_F_OUT)) || (need_dgw && !dgw_by_dots);
   if (want_q && query_rows(d) > 0) {
     aknn::QueryLaunch L{};
     L.dtype = d->dtype;
@@ -999,6 +1005,8 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.order = cache ? cache->query_order : nullptr;
     L.rec = cache ? cache->point_records : nullptr;
     L.g_bcast = bcast;
+    L.gw = gw;
+    L.dgw = (need_dgw && !dgw_by_dots) ? dgw : nullptr;
     hipError_t e;
     {
       Span sp(full ? "query_grad2_full" : (second ? "query_grad2" : "query_grad"), st);
@@ -1049,6 +1057,7 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.dq_contrib = via_contrib ? ws : nullptr;
     L.g_bcast = bcast;
     L.cl = clouds_of(d);
+    L.gw = gw;
     hipError_t e;
     {
       Span sp(full ? "point_grad2_full" : (second ? "point_grad2" : "point_grad"), st);
@@ -1100,12 +1109,38 @@ int aknn_point_dots(const aknn_dims* d, const void* grad_rows, const void* f, co
   return AKNN_OK;
 }
 
+int aknn_backward_w(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* grad_w_out,
+                    const void* q, const void* p, const void* f, const void* sigma, const void* R,
+                    const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
+                    const aknn_cache* cache, void* ws, size_t ws_bytes, void* dq, void* dp, void* df,
+                    void* dsigma, void* dR, void* stream) {
+  return backward_common(d, needs & ~(AKNN_NEED_DGRAD_F_OUT | AKNN_SECOND_FULL | AKNN_NEED_DGRAD_W_OUT), false,
+                         nullptr, Upstream2{}, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets, csr_entries,
+                         cache, ws, ws_bytes, nullptr, dq, dp, df, dsigma, dR, stream, grad_w_out, nullptr);
+}
+
+int aknn_backward2_w(const aknn_dims* d, uint32_t needs, const void* grad_grad_q, const void* grad_grad_p,
+                     const void* grad_grad_f, const void* grad_grad_sigma, const void* grad_grad_R,
+                     const void* grad_f_out, const void* grad_w_out, const void* q, const void* p,
+                     const void* f, const void* sigma, const void* R, const int64_t* idxs,
+                     const int32_t* csr_offsets, const int32_t* csr_entries, const aknn_cache* cache, void* ws,
+                     size_t ws_bytes, void* d_grad_f_out, void* d_grad_w_out, void* dq, void* dp, void* df,
+                     void* dsigma, void* dR, void* stream) {
+  const Upstream2 up{grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R};
+  if (!grad_grad_q && !up.any()) return fail(AKNN_E_INVALID, "every upstream gradient is NULL");
+  const uint32_t mode = up.any() ? (needs | AKNN_SECOND_FULL) : (needs & ~AKNN_SECOND_FULL);
+  return backward_common(d, mode, true, grad_grad_q, up, grad_f_out, q, p, f, sigma, R, idxs, csr_offsets,
+                         csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp, df, dsigma, dR, stream,
+                         grad_w_out, d_grad_w_out);
+}
+
 int aknn_backward(const aknn_dims* d, uint32_t needs, const void* grad_f_out, const void* q,
                   const void* p, const void* f, const void* sigma, const void* R,
                   const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
                   const aknn_cache* cache, void* ws, size_t ws_bytes, void* dq, void* dp, void* df,
                   void* dsigma, void* dR, void* stream) {
-  return backward_common(d, needs & ~(AKNN_NEED_DGRAD_F_OUT | AKNN_SECOND_FULL), false, nullptr, Upstream2{},
+  return backward_common(d, needs & ~(AKNN_NEED_DGRAD_F_OUT | AKNN_SECOND_FULL | AKNN_NEED_DGRAD_W_OUT), false,
+                         nullptr, Upstream2{},
                          grad_f_out, q, p, f, sigma, R,
                          idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, nullptr, dq, dp, df,
                          dsigma, dR, stream);
@@ -1118,7 +1153,8 @@ int aknn_backward2(const aknn_dims* d, uint32_t needs, const void* grad_grad_q,
                    const aknn_cache* cache, void* ws, size_t ws_bytes, void* d_grad_f_out,
                    void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
   if (!grad_grad_q) return fail(AKNN_E_INVALID, "grad_grad_q is NULL");
-  return backward_common(d, needs & ~AKNN_SECOND_FULL, true, grad_grad_q, Upstream2{}, grad_f_out, q, p, f,
+  return backward_common(d, needs & ~(AKNN_SECOND_FULL | AKNN_NEED_DGRAD_W_OUT), true, grad_grad_q, Upstream2{},
+                         grad_f_out, q, p, f,
                          sigma, R, idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp,
                          df, dsigma, dR, stream);
 }
@@ -1132,7 +1168,8 @@ int aknn_backward2_full(const aknn_dims* d, uint32_t needs, const void* grad_gra
                         void* dq, void* dp, void* df, void* dsigma, void* dR, void* stream) {
   const Upstream2 up{grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R};
   if (!grad_grad_q && !up.any()) return fail(AKNN_E_INVALID, "every upstream gradient is NULL");
-  return backward_common(d, needs | AKNN_SECOND_FULL, true, grad_grad_q, up, grad_f_out, q, p, f, sigma, R,
+  return backward_common(d, (needs | AKNN_SECOND_FULL) & ~AKNN_NEED_DGRAD_W_OUT, true, grad_grad_q, up,
+                         grad_f_out, q, p, f, sigma, R,
                          idxs, csr_offsets, csr_entries, cache, ws, ws_bytes, d_grad_f_out, dq, dp, df, dsigma,
                          dR, stream);
 }

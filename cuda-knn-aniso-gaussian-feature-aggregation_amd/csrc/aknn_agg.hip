@@ -204,6 +204,11 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
       }
     }
 
+    if constexpr (MODE == kBwd2 || MODE == kBwd2F) {
+      // d grad_w_out of slot k: the weight's upstream enters like grad_f_out . f[n_k]
+      if (L.dgw && have && row_ok) static_cast<T*>(L.dgw)[row * K + k] = coef;
+    }
+
     // All 16 row gathers of the batch are issued back to back: no per-lane branch in
     // this loop (lanes past F read a clamped in-row address and mask the value).
     int nj[16];
@@ -235,7 +240,8 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     });
 
     if (kBack && want_dq) {
-      const T a = row_transpose_sum<T>(part, r);  // a for slot k
+      T a = row_transpose_sum<T>(part, r);  // a for slot k
+      if (L.gw && have) a += static_cast<const T*>(L.gw)[row * K + k];  // grad_w_out path
       if (have) {
         if constexpr (MODE == kBwd1) {
           const T cw = T(-2) * a * w;  // dq = -2 sum a w v
@@ -465,6 +471,7 @@ __global__ __launch_bounds__(256) void dq_dots_kernel(QueryLaunch L) {
     Pt<T> pt;
     T a;
     wave_records<T>(dots, gp, stage, pt, a);
+    if (L.gw && have) a += static_cast<const T*>(L.gw)[row * K + k];  // grad_w_out path
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
     const T aw = a * w;
@@ -475,6 +482,7 @@ __global__ __launch_bounds__(256) void dq_dots_kernel(QueryLaunch L) {
       c2 = T(-2) * aw * geo.v2;
     } else {
       const T s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
+      if (L.dgw && have && row_ok) static_cast<T*>(L.dgw)[row * K + k] = T(-2) * w * s;  // d grad_w_out
       T hr[3];  // h . R_i
 #pragma unroll
       for (int i = 0; i < 3; ++i) hr[i] = pt.r[3 * i] * hx + pt.r[3 * i + 1] * hy + pt.r[3 * i + 2] * hz;

@@ -83,6 +83,8 @@ struct QueryLaunch {
   bool accumulate;         // forward: add into f_out instead of overwriting it
   bool g_bcast;            // grad_f_out holds one row per cloud ([B, F]) instead of per query
   const void* dots;        // dq from point dot products: a[n] = grad_f_out[cloud(n)] . f[n]
+  const void* gw;          // optional grad_w_out [rows][K]: added to each slot's a
+  void* dgw;               // optional second-order output d grad_w_out [rows][K]
 };
 
 // Point-side work over the CSR transpose of idxs.
@@ -103,6 +105,7 @@ struct PointLaunch {
   bool second;              // second order (grad_grad_q and the gg_* may each be null)
   bool g_bcast;             // grad_f_out holds one row per cloud ([B, F]) instead of per query
   Clouds cl;                // cloud of a point row (broadcast upstream)
+  const void* gw;           // optional grad_w_out [query rows * K]: added to each entry's a
 };
 
 template <typename T>

@@ -214,6 +214,7 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
       });
       a = row_transpose_sum<T>(part, r);
     }
+    if (L.gw && have) a += static_cast<const T*>(L.gw)[entry];  // grad_w_out path
 
     if (have) {
       const T u[3] = {geo.u0, geo.u1, geo.u2};
@@ -439,7 +440,8 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
       part[J.value] = dot;
       if constexpr (FULL) part2[J.value] = dot2;
     });
-    const T a = row_transpose_sum<T>(part, r);
+    T a = row_transpose_sum<T>(part, r);
+    if (L.gw && have) a += static_cast<const T*>(L.gw)[entry];  // grad_w_out path
     T ghf = T(0);  // g[q] . gg_f[n]
     if constexpr (FULL) {
       if (has_hf) ghf = row_transpose_sum<T>(part2, r);

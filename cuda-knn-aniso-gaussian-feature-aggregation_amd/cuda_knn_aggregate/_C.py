@@ -593,14 +593,25 @@ def _bwd_common(q, p, f, sigma, R, k_idxs, grads, gnames):
     return dt, B, Q, P, F, K
 
 
+def _check_gw(gw, B, Q, K, dt, name="grad_w_out"):
+    if gw is None:
+        return None
+    _check_input(gw, name)
+    if gw.dtype != dt or tuple(gw.shape) != (B, Q, K):
+        raise RuntimeError(f"{name} must be {dt} [B, Q, K] = {[B, Q, K]} (got {gw.dtype} {tuple(gw.shape)})")
+    return gw.contiguous()
+
+
 def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, k_idxs, *,
-                                 key: str = "aniso", csr=None, needs=None, cache=None, check: bool = True):
+                                 key: str = "aniso", csr=None, needs=None, cache=None, check: bool = True,
+                                 use_grad_w_out: bool = False):
     """First-order gradients -> ``[dq, dp, df, dsigma, dR]``.
 
     Reference: ``knn_aggregate_aniso_backward`` (csrc/knn_aggregate.cpp:101-132,
     csrc/backward.cu:331-389).  Like the reference, ``grad_w_out`` does not enter
-    (csrc/backward.cu:93-94); ``w_out`` is not read either -- weights are recomputed
-    from ``k_idxs``.  ``needs`` (5 bools) skips outputs (returned as ``None``);
+    (csrc/backward.cu:93-94) unless ``use_grad_w_out=True``: then ``grad_w_out`` [B,Q,K]
+    adds to every weight's upstream (``aknn_backward_w``).  ``w_out`` is not read --
+    weights are recomputed from ``k_idxs``.  ``needs`` (5 bools) skips outputs (returned as ``None``);
     ``csr`` = :func:`build_csr` result, built here when point gradients are needed.
     ``grad_f_out`` may be a broadcast view (:func:`broadcast_rows`).  ``k_idxs`` is
     range-checked unless ``check=False`` (the kernels trust it; the autograd Functions
@@ -613,6 +624,7 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
         raise RuntimeError(f"grad_f_out must have shape [B, Q, F] (got {tuple(grad_f_out.shape)})")
     names = ("dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    gw = _check_gw(grad_w_out, B, Q, K, dt) if use_grad_w_out else None
     if B * Q == 0:
         return _no_queries(names, mask, B, Q, P, F, dt, q.device)
     if check:
@@ -624,9 +636,11 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
         df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
         res = None
         for s0, s1 in slabs:
-            o = knn_aggregate_aniso_backward(_gcols(grad_f_out, rows, s0, s1), grad_w_out, q, p, _cols(f, s0, s1),
-                                             sigma, R, w_out, k_idxs, key=key, csr=csr, needs=needs, cache=cache,
-                                             check=False)
+            # the weights' own upstream enters once (with the first column range)
+            o = knn_aggregate_aniso_backward(_gcols(grad_f_out, rows, s0, s1), gw if s0 == 0 else None, q, p,
+                                             _cols(f, s0, s1), sigma, R, w_out, k_idxs, key=key, csr=csr,
+                                             needs=needs, cache=cache, check=False,
+                                             use_grad_w_out=gw is not None and s0 == 0)
             if df is not None:
                 df[..., s0:s1] = o[2]
             if res is None:
@@ -657,20 +671,30 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     with torch.cuda.device(dev):
-        _lib.check(lib.aknn_backward(ctypes.byref(dims), mask, grad_f_out.data_ptr(), q.data_ptr(),
-                                     p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
-                                     k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
-                                     P_(ws), ws_bytes, P_(outs["dq"]),
-                                     P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
-                                     P_(outs["dR"]), _stream(dev)))
+        if gw is not None:
+            _lib.check(lib.aknn_backward_w(ctypes.byref(dims), mask, grad_f_out.data_ptr(), gw.data_ptr(),
+                                           q.data_ptr(), p.data_ptr(), f.data_ptr(), sigma.data_ptr(),
+                                           R.data_ptr(), k_idxs.data_ptr(), P_(off), P_(ent),
+                                           _cp(_cache_struct(cache)), P_(ws), ws_bytes, P_(outs["dq"]),
+                                           P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]), P_(outs["dR"]),
+                                           _stream(dev)))
+        else:
+            _lib.check(lib.aknn_backward(ctypes.byref(dims), mask, grad_f_out.data_ptr(), q.data_ptr(),
+                                         p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                                         k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)),
+                                         P_(ws), ws_bytes, P_(outs["dq"]),
+                                         P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
+                                         P_(outs["dR"]), _stream(dev)))
     return [outs[n] for n in names]
 
 
 def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out,
                                      k_idxs, *, key: str = "aniso", csr=None, needs=None, cache=None,
                                      grad_grad_p=None, grad_grad_f=None, grad_grad_sigma=None,
-                                     grad_grad_R=None, check: bool = True):
-    """Second-order gradients -> ``[dgrad_f_out, dq, dp, df, dsigma, dR]``.
+                                     grad_grad_R=None, check: bool = True, use_grad_w_out: bool = False,
+                                     need_dgrad_w_out: bool = False):
+    """Second-order gradients -> ``[dgrad_f_out, dq, dp, df, dsigma, dR]`` (and
+    ``dgrad_w_out`` [B,Q,K] appended with ``use_grad_w_out=True``).
 
     Reference: ``knn_aggregate_aniso_backward_2nd`` (csrc/knn_aggregate.cpp:135-172,
     csrc/backward.cu:392-455), the gradients of ``<grad_grad_q, dq>``.  The keyword
@@ -678,7 +702,9 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
     and ``grad_grad_R`` [B,P,3,3] add the terms the reference drops
     (cuda_knn_aggregate/__init__.py:271-289); ``grad_grad_q`` may then be ``None``.
     ``needs`` (6 bools) skips outputs (``None``).  ``grad_f_out`` and ``check`` as in
-    :func:`knn_aggregate_aniso_backward`.
+    :func:`knn_aggregate_aniso_backward`.  ``use_grad_w_out=True``: the second order of the
+    first order with ``grad_w_out`` (``aknn_backward2_w``); the appended ``dgrad_w_out`` is
+    computed when ``need_dgrad_w_out`` (else ``None``).
     """
     ups = {"grad_grad_p": (grad_grad_p, 3), "grad_grad_f": (grad_grad_f, None),
            "grad_grad_sigma": (grad_grad_sigma, 3), "grad_grad_R": (grad_grad_R, (3, 3))}
@@ -702,8 +728,12 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
             raise RuntimeError(f"{n} must have shape {list(want)} (got {tuple(t.shape)})")
     names = ("dgf", "dq", "dp", "df", "dsigma", "dR")
     mask = _needs_mask(needs, names)
+    gw = _check_gw(grad_w_out, B, Q, K, dt) if use_grad_w_out else None
+    want_dgw = use_grad_w_out and need_dgrad_w_out
     if B * Q == 0:
-        return _no_queries(names, mask, B, Q, P, F, dt, q.device)
+        res = _no_queries(names, mask, B, Q, P, F, dt, q.device)
+        return res + [torch.zeros(B, Q, K, dtype=dt, device=q.device) if want_dgw else None] if use_grad_w_out \
+            else res
     if check:
         _check_range(k_idxs, P, "k_idxs")
     slabs = _feature_slabs(F, dt)
@@ -714,12 +744,14 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
         res = None
         for s0, s1 in slabs:
+            # the weights' own upstream, and its gradient, enter once (first column range)
+            first = s0 == 0
             o = knn_aggregate_aniso_backward_2nd(
-                grad_grad_q, _gcols(grad_f_out, rows, s0, s1), grad_w_out, q, p, _cols(f, s0, s1), sigma, R, w_out,
-                k_idxs,
+                grad_grad_q, _gcols(grad_f_out, rows, s0, s1), gw if first else None, q, p, _cols(f, s0, s1), sigma,
+                R, w_out, k_idxs,
                 key=key, csr=csr, needs=needs, cache=cache, grad_grad_p=grad_grad_p,
                 grad_grad_f=_cols(grad_grad_f, s0, s1), grad_grad_sigma=grad_grad_sigma, grad_grad_R=grad_grad_R,
-                check=False)
+                check=False, use_grad_w_out=use_grad_w_out and first, need_dgrad_w_out=want_dgw and first)
             if dgf is not None:
                 dgf[..., s0:s1] = o[0]
             if df is not None:
@@ -752,6 +784,17 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         grad_f_out = rows
     ws_bytes = lib.aknn_backward_workspace_size(ctypes.byref(dims), mask | (_lib.SECOND_FULL if full else 0))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
+    if use_grad_w_out:
+        dgw = torch.empty(B, Q, K, dtype=dt, device=dev) if want_dgw else None
+        with torch.cuda.device(dev):
+            _lib.check(lib.aknn_backward2_w(
+                ctypes.byref(dims), mask | (_lib.NEED_DGRAD_W_OUT if want_dgw else 0), P_(grad_grad_q),
+                P_(grad_grad_p), P_(grad_grad_f), P_(grad_grad_sigma), P_(grad_grad_R), grad_f_out.data_ptr(),
+                P_(gw), q.data_ptr(), p.data_ptr(), f.data_ptr(), sigma.data_ptr(), R.data_ptr(),
+                k_idxs.data_ptr(), P_(off), P_(ent), _cp(_cache_struct(cache)), P_(ws), ws_bytes,
+                P_(outs["dgf"]), P_(dgw), P_(outs["dq"]), P_(outs["dp"]), P_(outs["df"]), P_(outs["dsigma"]),
+                P_(outs["dR"]), _stream(dev)))
+        return [outs[n] for n in names] + [dgw]
     if full:
         with torch.cuda.device(dev):
             _lib.check(lib.aknn_backward2_full(

@@ -99,6 +99,12 @@ def knn_aggregate_aniso_lookup(
 # (a function of dq alone) gets the same result either way.
 FULL_SECOND_ORDER = True
 
+# Gradients arriving on w_out: False (the reference's behaviour, csrc/backward.cu:93-94
+# keeps "+ grad_w_out" commented out) runs the backward without them, so a loss on w_out
+# contributes nothing; True adds grad_w_out to each weight's upstream (first order) and
+# differentiates that term too (second order, including d grad_w_out).
+W_OUT_GRAD = False
+
 
 def _point_needs(flags):
     return any(flags)
@@ -132,9 +138,12 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         cache = tuple(ctx.saved_tensors[7:10]) if getattr(ctx, "has_cache", False) else None
         # Only grad_f_out enters (grad_w_out and grad_dists are ignored, as in the
         # reference, csrc/backward.cu:93-94): no grad on f_out means no gradient at all.
-        if grad_f_out is None:
+        use_w = W_OUT_GRAD and grad_w_out is not None
+        if grad_f_out is None and not use_w:
             return (None,) * 8
-        if grad_w_out is None:  # never read: a broadcast zero instead of a Q x K fill
+        if grad_f_out is None:  # a loss on w_out alone
+            grad_f_out = q.new_zeros(()).expand(q.shape[0], q.shape[1], f.shape[2])
+        if not use_w:  # never read: a broadcast zero instead of a Q x K fill
             grad_w_out = w_out.new_zeros(()).expand_as(w_out)
         needs = tuple(ctx.needs_input_grad[:5])
         # w_out is detached: the backward never reads it, and an edge from the
@@ -143,7 +152,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
         # library reads its rows instead of a materialised [B, Q, F] copy (same results).
         grad_q, grad_p, grad_f, grad_sigma, grad_R = KNNAnisotropicAggregateBackward.apply(
             _C.upstream(grad_f_out),
-            grad_w_out,  # never read
+            grad_w_out,  # read only under W_OUT_GRAD
             q,
             p,
             f,
@@ -154,6 +163,7 @@ class KNNAnisotropicAggregate(torch.autograd.Function):
             ctx.key,
             needs,
             cache,
+            use_w,
         )
         return grad_q, grad_p, grad_f, grad_sigma, grad_R, None, None, None
 
@@ -207,7 +217,7 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key="aniso",
-                needs=None, cache=None):
+                needs=None, cache=None, use_w=False):
         if needs is None:
             needs = (True,) * 5
         P = p.shape[1]
@@ -215,7 +225,8 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) and has_q else None
         dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward(
             grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs,
-            cache=cache, check=False)
+            cache=cache, check=False, use_grad_w_out=use_w)
+        ctx.use_w = use_w
         saved = [grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs]
         if csr is not None:
             saved += list(csr)
@@ -238,19 +249,20 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         csr = (rest.pop(0), rest.pop(0)) if ctx.has_csr else None
         cache = tuple(rest[:3]) if ctx.has_cache else None
         ni = ctx.needs_input_grad
-        # outputs: d grad_f_out, dq, dp, df, dsigma, dR
+        # outputs: d grad_f_out, dq, dp, df, dsigma, dR (and d grad_w_out on the grad_w_out path)
         needs = (ni[0], ni[2], ni[3], ni[4], ni[5], ni[6])
+        need_dgw = ctx.use_w and ni[1]
         ups = (grad_grad_p, grad_grad_f, grad_grad_sigma, grad_grad_R) if FULL_SECOND_ORDER else (None,) * 4
         ups = tuple(None if u is None else u.contiguous() for u in ups)
         full = any(u is not None for u in ups)
-        if (grad_grad_q is None and not full) or not any(needs):
-            return (None,) * 12
+        if (grad_grad_q is None and not full) or not (any(needs) or need_dgw):
+            return (None,) * 13
         if csr is None and (_point_needs(needs[2:]) or (full and needs[1])) and q.shape[0] * q.shape[1] > 0:
             csr = _C.build_csr(idxs, p.shape[1], ctx.key)
-        d_gfo, dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward_2nd(
+        res = _C.knn_aggregate_aniso_backward_2nd(
             None if grad_grad_q is None else grad_grad_q.contiguous(),
             _C.upstream(grad_f_out),
-            grad_w_out,  # never read
+            grad_w_out,  # read only on the grad_w_out path
             q,
             p,
             f,
@@ -267,5 +279,9 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
             grad_grad_sigma=ups[2],
             grad_grad_R=ups[3],
             check=False,
+            use_grad_w_out=ctx.use_w,
+            need_dgrad_w_out=need_dgw,
         )
-        return d_gfo, None, dq, dp, df, dsigma, dR, None, None, None, None, None
+        d_gfo, dq, dp, df, dsigma, dR = res[:6]
+        d_gwo = res[6] if ctx.use_w else None
+        return d_gfo, d_gwo, dq, dp, df, dsigma, dR, None, None, None, None, None, None

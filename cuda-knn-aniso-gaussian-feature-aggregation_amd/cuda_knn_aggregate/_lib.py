@@ -36,6 +36,7 @@ NEED_DR = 16
 NEED_DGRAD_F_OUT = 32
 SECOND_FULL = 64
 GRAD_BROADCAST = 128  # grad_f_out given as one row per cloud ([B, F])
+NEED_DGRAD_W_OUT = 256  # aknn_backward2_w: d grad_w_out
 NEED_POINTS = NEED_DP | NEED_DF | NEED_DSIGMA | NEED_DR
 
 EXPORTS = (
@@ -53,6 +54,8 @@ EXPORTS = (
     "aknn_backward",
     "aknn_backward2",
     "aknn_backward2_full",
+    "aknn_backward_w",
+    "aknn_backward2_w",
     "aknn_index_size",
     "aknn_index_workspace_size",
     "aknn_index_build",
@@ -138,6 +141,10 @@ _SIGS = {
                        + [_P] * 6 + [_P]),
     "aknn_backward2_full": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 14 + [_CACHE, _P, ctypes.c_size_t]
                             + [_P] * 6 + [_P]),
+    "aknn_backward_w": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 10 + [_CACHE, _P, ctypes.c_size_t]
+                        + [_P] * 5 + [_P]),
+    "aknn_backward2_w": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 15 + [_CACHE, _P, ctypes.c_size_t]
+                         + [_P] * 7 + [_P]),
     "aknn_index_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_index_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_index_build": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P]),

@@ -33,12 +33,12 @@ import torch
 from . import _C
 
 CANDIDATES = (
-    [{"search": "tiles", "sub_records": r, "super_tiles": n} for r in (8, 4, 16) for n in (16, 8, 32)]
+    [{"search": "tiles", "sub_records": r, "super_tiles": n} for r in (8, 4, 16) for n in (8, 16, 32)]
     + [{"search": "tiles", "sub_records": r, "super_tiles": n, "mu_classes": c}
-       for c in (2, 4) for r in (8, 4) for n in (16, 8)]
+       for c in (2, 4) for r in (8, 4) for n in (8, 16)]
     + [{"search": "tiles", "waves_per_eu": w} for w in (4, 8)]
-    + [{"search": "tiles", "persistent": s, "super_tiles": n} for s in (1, -1) for n in (16, 8)]
-    + [{"search": "wave", "super_tiles": n} for n in (16, 8, 32)]
+    + [{"search": "tiles", "persistent": s, "super_tiles": n} for s in (1, -1) for n in (8, 16)]
+    + [{"search": "wave", "super_tiles": n} for n in (8, 16, 32)]
     + [{"search": "brute", "queries_per_lane": q, "point_split": s} for q in (1, 2, 4) for s in (1, 2, 4, 8)]
 )
 

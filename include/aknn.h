@@ -260,6 +260,33 @@ AKNN_API int aknn_backward2_full(const aknn_dims* dims, uint32_t needs, const vo
                                  void* d_grad_f_out, void* dq, void* dp, void* df, void* dsigma,
                                  void* dR, void* stream);
 
+/* The grad_w_out path (no counterpart in the reference, whose first-order kernel keeps
+ * "+ grad_w_out" commented out at csrc/backward.cu:93-94, so a loss on w_out contributes
+ * nothing there; the default path here does the same).  With grad_w_out [query rows, K]
+ * (same dtype as the inputs; NULL = zero) the upstream of the weight w_k of slot k is
+ * grad_f_out . f[n_k] + grad_w_out[k] instead of its first term: every dq, dp, dsigma and
+ * dR term gains it, df does not (w_out does not depend on f).
+ *   aknn_backward_w: as aknn_backward (broadcast upstream allowed).
+ *   aknn_backward2_w: the second order of that map, as aknn_backward2 when grad_grad_p
+ *   .. grad_grad_R are all NULL, else as aknn_backward2_full (materialised grad_f_out,
+ *   workspace for needs | AKNN_SECOND_FULL); with AKNN_NEED_DGRAD_W_OUT in needs it also
+ *   writes d_grad_w_out [query rows, K], the gradient with respect to grad_w_out. */
+#define AKNN_NEED_DGRAD_W_OUT 256u
+AKNN_API int aknn_backward_w(const aknn_dims* dims, uint32_t needs, const void* grad_f_out,
+                             const void* grad_w_out, const void* q, const void* p, const void* f,
+                             const void* sigma, const void* R, const int64_t* idxs,
+                             const int32_t* csr_offsets, const int32_t* csr_entries, const aknn_cache* cache,
+                             void* workspace, size_t workspace_bytes, void* dq, void* dp, void* df,
+                             void* dsigma, void* dR, void* stream);
+AKNN_API int aknn_backward2_w(const aknn_dims* dims, uint32_t needs, const void* grad_grad_q,
+                              const void* grad_grad_p, const void* grad_grad_f, const void* grad_grad_sigma,
+                              const void* grad_grad_R, const void* grad_f_out, const void* grad_w_out,
+                              const void* q, const void* p, const void* f, const void* sigma, const void* R,
+                              const int64_t* idxs, const int32_t* csr_offsets, const int32_t* csr_entries,
+                              const aknn_cache* cache, void* workspace, size_t workspace_bytes,
+                              void* d_grad_f_out, void* d_grad_w_out, void* dq, void* dp, void* df,
+                              void* dsigma, void* dR, void* stream);
+
 /* Table-lookup forward.  Replaces knn_lookup_aggregate_aniso_forward
  * (csrc/knn_aggregate.cpp:50-73; kernel csrc/forward.cu:113-183, launcher :341-395).  For
  * each query: n1 = its Euclidean nearest point (the lowest index reaching the minimum of

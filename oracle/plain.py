@@ -191,17 +191,20 @@ def _scatter_points(B, P, idxs, val):
     return out.reshape((B, P) + tail)
 
 
-def backward_closed(g, q, p, f, sigma, R, idxs):
+def backward_closed(g, q, p, f, sigma, R, idxs, gw=None):
     """First-order gradients by closed form: csrc/backward.cu:31-120.
 
     With ``a = g . f[n]``: ``dq = -2 sum a w v``; ``dp[n] += 2 a w v``;
     ``dsigma[n]_i += -a w u_i^2``; ``dR[n]_ij += -2 a w s_i u_i d_j``; ``df[n] += w g``.
-    ``grad_w_out`` does not enter (csrc/backward.cu:93-94).
+    ``grad_w_out`` does not enter (csrc/backward.cu:93-94) unless given as ``gw``
+    [B,Q,K]: then ``a = g . f[n] + gw`` (the weight's whole upstream; ``df`` unchanged).
     """
     B, P = p.shape[0], p.shape[1]
     d, u, D, w, v, sn, Rn = _geometry(q, p, sigma, R, idxs)
     fn = gather_points(f, idxs)
     a = (g[:, :, None, :] * fn).sum(-1)             # [B,Q,K]
+    if gw is not None:
+        a = a + gw
     aw = a * w
     dq = -2.0 * (aw[..., None] * v).sum(2)
     dp = _scatter_points(B, P, idxs, 2.0 * aw[..., None] * v)
@@ -211,7 +214,7 @@ def backward_closed(g, q, p, f, sigma, R, idxs):
     return dq, dp, df, dsig, dR
 
 
-def backward2_closed(h, g, q, p, f, sigma, R, idxs):
+def backward2_closed(h, g, q, p, f, sigma, R, idxs, gw=None):
     """Second-order gradients of ``<h, dq>``: csrc/backward.cu:152-327.
 
     With ``s = h . v``, ``M = R^T diag(s) R`` and ``t = 2 s v - M h``:
@@ -219,12 +222,16 @@ def backward2_closed(h, g, q, p, f, sigma, R, idxs):
     ``dp[n] += -2 a w t``; ``dsigma[n]_i += 2 a w u_i (u_i s - h.R_i)``;
     ``dR[n]_ij += -2 a w s_i (h_j u_i + (h.R_i) d_j - 2 s u_i d_j)``.
     Returns ``(d_grad_f_out, dq, dp, df, dsigma, dR)`` like the reference's
-    ``knn_aggregate_aniso_backward_2nd`` (csrc/backward.cu:454).
+    ``knn_aggregate_aniso_backward_2nd`` (csrc/backward.cu:454).  With ``gw`` (the
+    first order's ``grad_w_out``): ``a = g . f[n] + gw`` in every ``a w`` term, and
+    ``d_grad_w_out = -2 w s`` is appended.
     """
     B, P = p.shape[0], p.shape[1]
     d, u, D, w, v, sn, Rn = _geometry(q, p, sigma, R, idxs)
     fn = gather_points(f, idxs)
     a = (g[:, :, None, :] * fn).sum(-1)
+    if gw is not None:
+        a = a + gw
     hk = h[:, :, None, :]
     s = (hk * v).sum(-1)                            # [B,Q,K]
     Mh = torch.einsum("bqkij,bqkj->bqki", Rn.transpose(-1, -2),
@@ -241,6 +248,8 @@ def backward2_closed(h, g, q, p, f, sigma, R, idxs):
              + hR[..., :, None] * d[..., None, :]
              - 2.0 * s[..., None, None] * u[..., :, None] * d[..., None, :])
     dR = _scatter_points(B, P, idxs, -2.0 * aw[..., None, None] * sn[..., :, None] * inner)
+    if gw is not None:
+        return dg, dq, dp, df, dsig, dR, -2.0 * w * s
     return dg, dq, dp, df, dsig, dR
 
 
@@ -278,6 +287,28 @@ def backward2_full_autograd(h, hp, hf, hs, hR, g, q, p, f, sigma, R, idxs):
     first = torch.autograd.grad(out, ins, gg, create_graph=True)
     G = sum((d * u).sum() for d, u in zip(first, (h, hp, hf, hs, hR)) if u is not None)
     return torch.autograd.grad(G, [gg] + ins, allow_unused=True)
+
+
+def backward_w_autograd(g, gw, q, p, f, sigma, R, idxs):
+    """First-order gradients of ``<g, f_out> + <gw, w_out>`` by autograd of the plain
+    formulation (the grad_w_out path)."""
+    ins = [t.detach().requires_grad_(True) for t in (q, p, f, sigma, R)]
+    f_out, w, _ = forward_given_idxs(*ins, idxs)
+    return torch.autograd.grad((f_out * g).sum() + (w * gw).sum(), ins)
+
+
+def backward2_w_autograd(ups, g, gw, q, p, f, sigma, R, idxs):
+    """Second order of the grad_w_out path by double autograd: the gradients of
+    ``sum_i <ups[i], first[i]>`` (``first`` = (dq, dp, df, dsigma, dR) of
+    ``<g, f_out> + <gw, w_out>``; entries of ``ups`` may be ``None``) with respect to
+    ``(g, gw, q, p, f, sigma, R)``."""
+    ins = [t.detach().requires_grad_(True) for t in (q, p, f, sigma, R)]
+    gg = g.detach().requires_grad_(True)
+    gww = gw.detach().requires_grad_(True)
+    f_out, w, _ = forward_given_idxs(*ins, idxs)
+    first = torch.autograd.grad((f_out * gg).sum() + (w * gww).sum(), ins, create_graph=True)
+    G = sum((d * u).sum() for d, u in zip(first, ups) if u is not None)
+    return torch.autograd.grad(G, [gg, gww] + ins, allow_unused=True)
 
 
 def eikonal_step(q, p, f, sigma, R, K, v, key="aniso", select=knn_select):

@@ -388,3 +388,51 @@ def test_full_second_order_gradgradcheck():
     ups = [torch.randn_like(t) for t in ins]
     got = plain.backward2_full_autograd(*ups, g, *ins, idxs)
     assert all(x is not None for x in got)
+
+
+# ----------------------------------------------------------------------------- grad_w_out path
+def test_w_out_path_closed_forms_match_autograd():
+    """With grad_w_out, the closed forms (a = g.f[n] + gw) equal FP64 autograd of
+    <g, f_out> + <gw, w_out>, first and second order (d grad_w_out included)."""
+    b = small(sigma=(2.0, 20.0), seed=3)
+    _, idx = plain.knn_sort(b.q, b.p, b.sigma, b.R, 6)
+    ins = _f64(b)
+    g = torch.randn(2, 60, 5, dtype=torch.float64)
+    gw = torch.randn(2, 60, 6, dtype=torch.float64)
+    c = plain.backward_closed(g, *ins, idx, gw=gw)
+    a = plain.backward_w_autograd(g, gw, *ins, idx)
+    for x, y in zip(c, a):
+        assert plain.rel_err(x, y) < 1e-12
+    h = torch.randn(2, 60, 3, dtype=torch.float64)
+    c2 = plain.backward2_closed(h, g, *ins, idx, gw=gw)      # dg, dq, dp, df, ds, dR, dgw
+    a2 = plain.backward2_w_autograd((h, None, None, None, None), g, gw, *ins, idx)  # dg, dgw, dq..dR
+    order = [0, 6, 1, 2, 3, 4, 5]
+    for i, y in zip(order, a2):
+        assert plain.rel_err(c2[i], y) < 1e-12, i
+    # gw = 0 gives the reference's gradients
+    z = plain.backward_closed(g, *ins, idx, gw=torch.zeros_like(gw))
+    for x, y in zip(z, plain.backward_closed(g, *ins, idx)):
+        assert plain.rel_err(x, y) < 1e-15
+
+
+def test_w_out_path_central_differences():
+    """d<gw, w_out>/dx by central differences of the FP64 forward."""
+    b = small(B=1, Q=8, P=6, K=6, F=3, sigma=(1.0, 5.0), seed=4)
+    ins = _f64(b)
+    idx = plain.knn_sort(*[b.q, b.p, b.sigma, b.R], 6)[1]
+    g = torch.zeros(1, 8, 3, dtype=torch.float64)
+    gw = torch.randn(1, 8, 6, dtype=torch.float64)
+    grads = plain.backward_closed(g, *ins, idx, gw=gw)
+    eps = 1e-6
+    for which in (0, 1, 3, 4):
+        flat = ins[which].reshape(-1)
+        for j in range(0, flat.numel(), max(1, flat.numel() // 5)):
+            old = flat[j].item()
+            flat[j] = old + eps
+            fp = (plain.forward_given_idxs(*ins, idx)[1] * gw).sum().item()
+            flat[j] = old - eps
+            fm = (plain.forward_given_idxs(*ins, idx)[1] * gw).sum().item()
+            flat[j] = old
+            an = grads[which].reshape(-1)[j].item()
+            assert abs((fp - fm) / (2 * eps) - an) <= 1e-6 * max(1.0, abs(an)), (which, j)
+    assert float(grads[2].abs().max()) == 0.0  # w_out does not depend on f
