@@ -856,7 +856,7 @@ int aknn_lookup_forward(const aknn_dims* d, const void* q, const void* p, const 
 size_t aknn_csr_workspace_size(const aknn_dims* d) {
   if (check_dims(d, false) != AKNN_OK) return 0;
   size_t bytes = 0;
-  const hipError_t e = aknn::csr_temp_bytes(query_rows(d) * d->K, point_rows(d), &bytes);
+  const hipError_t e = aknn::csr_temp_bytes(query_rows(d) * d->K, point_rows(d), query_rows(d), &bytes);
   if (e != hipSuccess) {
     hip_fail(e, "csr_workspace_size");
     return 0;
@@ -864,8 +864,8 @@ size_t aknn_csr_workspace_size(const aknn_dims* d) {
   return bytes;
 }
 
-int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets,
-                   int32_t* csr_entries, void* ws, size_t ws_bytes, void* stream) {
+static int build_csr_common(const aknn_dims* d, const int64_t* idxs, const int32_t* qorder, int32_t* csr_offsets,
+                            int32_t* csr_entries, void* ws, size_t ws_bytes, void* stream) {
   int rc = check_dims(d, true);
   if (rc) return rc;
   if (!csr_offsets || (query_rows(d) > 0 && (!idxs || !csr_entries)))
@@ -878,10 +878,21 @@ int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets
   {
     Span sp("build_csr", static_cast<hipStream_t>(stream));
     e = aknn::launch_build_csr(idxs, clouds_of(d), query_rows(d), d->K, point_rows(d), csr_offsets,
-                               csr_entries, ws, ws_bytes, static_cast<hipStream_t>(stream));
+                               csr_entries, ws, ws_bytes, static_cast<hipStream_t>(stream), qorder);
   }
   if (e != hipSuccess) return hip_fail(e, "build_csr");
   return AKNN_OK;
+}
+
+int aknn_build_csr(const aknn_dims* d, const int64_t* idxs, int32_t* csr_offsets, int32_t* csr_entries, void* ws,
+                   size_t ws_bytes, void* stream) {
+  return build_csr_common(d, idxs, nullptr, csr_offsets, csr_entries, ws, ws_bytes, stream);
+}
+
+int aknn_build_csr_ordered(const aknn_dims* d, const int64_t* idxs, const int32_t* query_order,
+                           int32_t* csr_offsets, int32_t* csr_entries, void* ws, size_t ws_bytes, void* stream) {
+  if (!query_order && d && query_rows(d) > 0) return fail(AKNN_E_INVALID, "query_order is NULL");
+  return build_csr_common(d, idxs, query_order, csr_offsets, csr_entries, ws, ws_bytes, stream);
 }
 
 // dq via per-entry contributions when the point kernels run anyway.
@@ -898,6 +909,18 @@ struct Upstream2 {  // the second order's upstream beyond grad_grad_q (full seco
 // Bytes of the point dot products of a broadcast upstream.
 size_t dots_bytes(const aknn_dims* d) { return align256(static_cast<size_t>(point_rows(d)) * 16 * elem_size(d)); }
 
+// Workspace of the backward without the ordered query copies.
+static size_t backward_base_bytes(const aknn_dims* d, uint32_t needs) {
+  if (needs & AKNN_GRAD_BROADCAST) return (needs & AKNN_NEED_DQ) && query_rows(d) > 0 ? dots_bytes(d) : 0;
+  if (!contrib_path(d, needs)) return 0;
+  return align256(static_cast<size_t>(query_rows(d)) * d->K * 3 * elem_size(d));
+}
+
+// Ordered copies of the query coordinates and of grad_grad_q (CSR in the query order).
+static size_t ordered_copy_bytes(const aknn_dims* d) {
+  return align256(static_cast<size_t>(query_rows(d)) * 3 * elem_size(d));
+}
+
 static int backward_common(const aknn_dims* d, uint32_t needs, bool second, const void* gg,
                            const Upstream2& up, const void* g,
                            const void* q, const void* p, const void* f, const void* sigma,
@@ -910,6 +933,10 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool need_dgw = second && (needs & AKNN_NEED_DGRAD_W_OUT) && query_rows(d) > 0;
   needs &= ~AKNN_NEED_DGRAD_W_OUT;
+  const bool csr_ordered = (needs & AKNN_CSR_QUERY_ORDER) != 0;
+  needs &= ~AKNN_CSR_QUERY_ORDER;
+  if (csr_ordered && query_rows(d) > 0 && !(cache_in && cache_in->query_order))
+    return fail(AKNN_E_INVALID, "AKNN_CSR_QUERY_ORDER needs the cache's query order");
   if (need_dgw && !dgw) return fail(AKNN_E_INVALID, "d_grad_w_out requested but NULL");
   const bool full = second && up.any();
   if (full && !(needs & AKNN_SECOND_FULL))
@@ -1033,6 +1060,21 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
   if (((needs & AKNN_NEED_POINTS) || via_contrib) && point_rows(d) > 0) {
     if (!csr_off || (query_rows(d) > 0 && !csr_ent))
       return fail(AKNN_E_INVALID, "point gradients need the CSR from aknn_build_csr");
+    // CSR in the spatial query order: the query-side rows the point kernels gather, in
+    // that order, after the workspace's other contents
+    void *q_sorted = nullptr, *hh_sorted = nullptr;
+    if (csr_ordered && query_rows(d) > 0) {
+      const size_t base = backward_base_bytes(d, needs | (full ? AKNN_SECOND_FULL : 0u));
+      const size_t total = base + 2 * ordered_copy_bytes(d);
+      if (!ws || ws_bytes < total)
+        return fail(AKNN_E_WORKSPACE, "workspace too small for AKNN_CSR_QUERY_ORDER (%zu < %zu bytes)", ws_bytes,
+                    total);
+      q_sorted = static_cast<char*>(ws) + base;
+      hh_sorted = static_cast<char*>(q_sorted) + ordered_copy_bytes(d);
+      hipError_t e = aknn::launch_gather3(q, cache->query_order, q_sorted, query_rows(d), d->dtype, st);
+      if (e == hipSuccess && gg) e = aknn::launch_gather3(gg, cache->query_order, hh_sorted, query_rows(d), d->dtype, st);
+      if (e != hipSuccess) return hip_fail(e, "query_order copies");
+    }
     aknn::PointLaunch L{};
     L.dtype = d->dtype;
     L.key = d->key;
@@ -1058,6 +1100,11 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
     L.g_bcast = bcast;
     L.cl = clouds_of(d);
     L.gw = gw;
+    if (q_sorted) {
+      L.qorder = cache->query_order;
+      L.q_sorted = q_sorted;
+      L.hh_sorted = gg ? hh_sorted : nullptr;
+    }
     hipError_t e;
     {
       Span sp(full ? "point_grad2_full" : (second ? "point_grad2" : "point_grad"), st);
@@ -1077,9 +1124,9 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
 
 size_t aknn_backward_workspace_size(const aknn_dims* d, uint32_t needs) {
   if (check_dims(d, false) != AKNN_OK) return 0;
-  if (needs & AKNN_GRAD_BROADCAST) return (needs & AKNN_NEED_DQ) && query_rows(d) > 0 ? dots_bytes(d) : 0;
-  if (!contrib_path(d, needs)) return 0;
-  return align256(static_cast<size_t>(query_rows(d)) * d->K * 3 * elem_size(d));
+  size_t bytes = backward_base_bytes(d, needs);
+  if (needs & AKNN_CSR_QUERY_ORDER) bytes += 2 * ordered_copy_bytes(d);
+  return bytes;
 }
 
 int aknn_point_dots(const aknn_dims* d, const void* grad_rows, const void* f, const void* p, const void* sigma,

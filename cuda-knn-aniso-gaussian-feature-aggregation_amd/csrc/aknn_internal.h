@@ -106,6 +106,11 @@ struct PointLaunch {
   bool g_bcast;             // grad_f_out holds one row per cloud ([B, F]) instead of per query
   Clouds cl;                // cloud of a point row (broadcast upstream)
   const void* gw;           // optional grad_w_out [query rows * K]: added to each entry's a
+  // CSR in a spatial query numbering (entries pos*K + k, qorder[pos] = query row): the
+  // query coordinates and grad_grad_q rows are read from copies in that order
+  const int32_t* qorder;
+  const void* q_sorted;
+  const void* hh_sorted;
 };
 
 template <typename T>
@@ -147,12 +152,18 @@ hipError_t launch_contrib_reduce(const void* contrib, int64_t rows, int64_t K, v
 size_t radix_sort_workspace(int64_t n, size_t key_bytes);
 template <typename K>
 hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-                            int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st);
+                            int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st,
+                            const int32_t* vrank = nullptr, uint32_t vk = 1);
 
 // CSR build
-hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes);
+hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, int64_t query_rows, size_t* bytes);
+// qorder (optional): entries number (position in qorder) * K + k instead of q * K + k
+// (each row's entries stay in ascending q * K + k order)
 hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
                             int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
-                            size_t ws_bytes, hipStream_t st);
+                            size_t ws_bytes, hipStream_t st, const int32_t* qorder = nullptr);
+// out[i][0..2] = src[order[i]][0..2] (query-side rows in spatial order)
+hipError_t launch_gather3(const void* src, const int32_t* order, void* out, int64_t rows, int32_t dtype,
+                          hipStream_t st);
 
 }  // namespace aknn

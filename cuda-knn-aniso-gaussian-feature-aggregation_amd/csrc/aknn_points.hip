@@ -112,6 +112,12 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   const int64_t F = L.F;
   const int K = static_cast<int>(L.K);
   const FastDiv kdiv(static_cast<uint32_t>(K));  // entry -> query row (entries < 2^31)
+  // CSR in the spatial query order: query-side inputs from the ordered copies, and the
+  // query row itself only where a per-query array is indexed by it
+  const bool ordered = L.qorder != nullptr;
+  const T* __restrict__ qsrc = ordered ? static_cast<const T*>(L.q_sorted) : q;
+  const T* __restrict__ hsrc = ordered ? static_cast<const T*>(L.hh_sorted) : hh;
+  const bool need_row = !BC || L.dq_contrib != nullptr || L.gw != nullptr;  // uniform
 
   const int r = threadIdx.x & 15;
   const int gq = (threadIdx.x >> 4) % G;
@@ -158,15 +164,19 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     const int e = eb + 16 * gq + r;
     const bool have = e < e1;
     const int entry = have ? ent[e] : 0;
-    const int64_t qrow = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
-    const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
+    // the entry's query: position pos of the CSR's numbering, row qrow, slot entry
+    // qrow*K + k (pos = qrow unless the CSR is in the spatial query order)
+    const int64_t pos = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
+    const int64_t qrow = (ordered && need_row) ? static_cast<int64_t>(L.qorder[pos]) : pos;
+    const int oentry = ordered ? static_cast<int>(qrow * K + (entry - pos * K)) : entry;
+    const T qx = qsrc[pos * 3 + 0], qy = qsrc[pos * 3 + 1], qz = qsrc[pos * 3 + 2];
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
     T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
     if constexpr (SECOND) {
-      hx = hh[qrow * 3 + 0];
-      hy = hh[qrow * 3 + 1];
-      hz = hh[qrow * 3 + 2];
+      hx = hsrc[pos * 3 + 0];
+      hy = hsrc[pos * 3 + 1];
+      hz = hsrc[pos * 3 + 2];
       s = hx * geo.v0 + hy * geo.v1 + hz * geo.v2;
       coef = T(-2) * w * s;  // df += -2 w s g
     } else {
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
       });
       a = row_transpose_sum<T>(part, r);
     }
-    if (L.gw && have) a += static_cast<const T*>(L.gw)[entry];  // grad_w_out path
+    if (L.gw && have) a += static_cast<const T*>(L.gw)[oentry];  // grad_w_out path
 
     if (have) {
       const T u[3] = {geo.u0, geo.u1, geo.u2};
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
       const T sg[3] = {pt.s0, pt.s1, pt.s2};
       const T aw = a * w;
       T* __restrict__ cq = static_cast<T*>(L.dq_contrib);
-      const int64_t cslot = static_cast<int64_t>(entry);
+      const int64_t cslot = static_cast<int64_t>(oentry);
       if constexpr (!SECOND) {
         acc[0] += T(2) * aw * geo.v0;
         acc[1] += T(2) * aw * geo.v1;
@@ -322,6 +332,12 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
   const int64_t F = L.F;
   const int K = static_cast<int>(L.K);
   const FastDiv kdiv(static_cast<uint32_t>(K));  // entry -> query row (entries < 2^31)
+  // CSR in the spatial query order: query-side inputs from the ordered copies, and the
+  // query row itself only where a per-query array is indexed by it
+  const bool ordered = L.qorder != nullptr;
+  const T* __restrict__ qsrc = ordered ? static_cast<const T*>(L.q_sorted) : q;
+  const T* __restrict__ hsrc = ordered ? static_cast<const T*>(L.hh_sorted) : hh;
+  const bool need_row = true;  // grad_f_out rows are gathered per entry
 
   const int r = threadIdx.x & 15;
   const int gq = (threadIdx.x >> 4) % G;
@@ -384,8 +400,12 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
     const int e = eb + 16 * gq + r;
     const bool have = e < e1;
     const int entry = have ? ent[e] : 0;
-    const int64_t qrow = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
-    const T qx = q[qrow * 3 + 0], qy = q[qrow * 3 + 1], qz = q[qrow * 3 + 2];
+    // the entry's query: position pos of the CSR's numbering, row qrow, slot entry
+    // qrow*K + k (pos = qrow unless the CSR is in the spatial query order)
+    const int64_t pos = have ? static_cast<int64_t>(kdiv.div(static_cast<uint32_t>(entry))) : 0;
+    const int64_t qrow = (ordered && need_row) ? static_cast<int64_t>(L.qorder[pos]) : pos;
+    const int oentry = ordered ? static_cast<int>(qrow * K + (entry - pos * K)) : entry;
+    const T qx = qsrc[pos * 3 + 0], qy = qsrc[pos * 3 + 1], qz = qsrc[pos * 3 + 2];
     const Geo<T> geo = geometry<T>(qx, qy, qz, pt);
     const T w = have ? geo.w : T(0);
     T hx = T(0), hy = T(0), hz = T(0), s = T(0), coef;
@@ -394,9 +414,9 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
       // grad_grad_q is always given outside the full second order: loaded with no
       // branch there, so these loads issue together with the query's
       if (!FULL || hh) {
-        hx = hh[qrow * 3 + 0];
-        hy = hh[qrow * 3 + 1];
-        hz = hh[qrow * 3 + 2];
+        hx = hsrc[pos * 3 + 0];
+        hy = hsrc[pos * 3 + 1];
+        hz = hsrc[pos * 3 + 2];
       }
       if constexpr (FULL) {
         sec = second_phi<T>(geo, pt, hx, hy, hz, up);
@@ -441,7 +461,7 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
       if constexpr (FULL) part2[J.value] = dot2;
     });
     T a = row_transpose_sum<T>(part, r);
-    if (L.gw && have) a += static_cast<const T*>(L.gw)[entry];  // grad_w_out path
+    if (L.gw && have) a += static_cast<const T*>(L.gw)[oentry];  // grad_w_out path
     T ghf = T(0);  // g[q] . gg_f[n]
     if constexpr (FULL) {
       if (has_hf) ghf = row_transpose_sum<T>(part2, r);
@@ -453,7 +473,7 @@ __global__ __launch_bounds__(256) void point_kernel_full(PointLaunch L) {
       const T sg[3] = {pt.s0, pt.s1, pt.s2};
       const T aw = a * w;
       T* __restrict__ cq = static_cast<T*>(L.dq_contrib);
-      const int64_t cslot = static_cast<int64_t>(entry);
+      const int64_t cslot = static_cast<int64_t>(oentry);
       if constexpr (!SECOND) {
         acc[0] += T(2) * aw * geo.v0;
         acc[1] += T(2) * aw * geo.v1;
@@ -813,6 +833,12 @@ __global__ void csr_keys_kernel(const int64_t* __restrict__ idxs, Clouds cl, int
   keys[e] = static_cast<uint32_t>(cl.p_begin(b) + idxs[e]);
 }
 
+// rank[order[i]] = i: a query row's position in the spatial order.
+__global__ void rank_kernel(const int32_t* __restrict__ order, int64_t n, int32_t* __restrict__ rank) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) rank[order[i]] = static_cast<int32_t>(i);
+}
+
 // Four entries per thread (K divisible by 4, so the four share a query row): one
 // 32-bit division and one cloud lookup for the four, 16-byte loads and store.  The
 // entry count is below 2^31 (the argument check), and idxs is 16-byte aligned.
@@ -851,15 +877,17 @@ static int key_bits(int64_t rows) {
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, size_t* bytes) {
+hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, int64_t query_rows, size_t* bytes) {
   (void)point_rows;
-  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(radix_sort_workspace(entries, 4));
+  // keys in and out, the sort's temporaries, the query ranks of an ordered transpose
+  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(radix_sort_workspace(entries, 4)) +
+           align256(static_cast<size_t>(query_rows) * 4);
   return hipSuccess;
 }
 
 hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
                             int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
-                            size_t ws_bytes, hipStream_t st) {
+                            size_t ws_bytes, hipStream_t st, const int32_t* qorder) {
   (void)ws_bytes;
   const int64_t entries = query_rows * K;
   char* base = static_cast<char*>(ws);
@@ -878,13 +906,45 @@ hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, 
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
+    // in the spatial query order: the sort's values (entry numbers, which keep each row's
+    // entries in ascending q*K+k order) become position*K + k through the rank array
+    int32_t* rank = nullptr;
+    if (qorder) {
+      rank = reinterpret_cast<int32_t*>(static_cast<char*>(tmp) + align256(radix_sort_workspace(entries, 4)));
+      hipLaunchKernelGGL(rank_kernel, dim3(static_cast<unsigned>((query_rows + 255) / 256)), dim3(256), 0, st,
+                         qorder, query_rows, rank);
+    }
     err = radix_sort_pairs<uint32_t>(keys_in, keys_out, nullptr, reinterpret_cast<uint32_t*>(csr_ent),
-                                     entries, 0, key_bits(point_rows), tmp, st);
+                                     entries, 0, key_bits(point_rows), tmp, st, rank, static_cast<uint32_t>(K));
     if (err != hipSuccess) return err;
   }
   const int bs = 256;
   hipLaunchKernelGGL(csr_offsets_kernel, dim3(static_cast<unsigned>((point_rows + 1 + bs - 1) / bs)),
                      dim3(bs), 0, st, keys_out, entries, point_rows, csr_off);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void gather3_kernel(const T* __restrict__ src, const int32_t* __restrict__ order, T* __restrict__ out,
+                               int64_t rows) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int64_t r = order[i];
+  out[i * 3 + 0] = src[r * 3 + 0];
+  out[i * 3 + 1] = src[r * 3 + 1];
+  out[i * 3 + 2] = src[r * 3 + 2];
+}
+
+hipError_t launch_gather3(const void* src, const int32_t* order, void* out, int64_t rows, int32_t dtype,
+                          hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((rows + 255) / 256)), block(256);
+  if (dtype == AKNN_F32)
+    hipLaunchKernelGGL((gather3_kernel<float>), grid, block, 0, st, static_cast<const float*>(src), order,
+                       static_cast<float*>(out), rows);
+  else
+    hipLaunchKernelGGL((gather3_kernel<double>), grid, block, 0, st, static_cast<const double*>(src), order,
+                       static_cast<double*>(out), rows);
   return hipGetLastError();
 }
 

@@ -133,7 +133,7 @@ template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift, int64_t tiles,
     const uint32_t* __restrict__ start, const uint32_t* __restrict__ totals, const uint32_t* __restrict__ tile_counts,
-    K* __restrict__ kout, uint32_t* __restrict__ vout) {
+    K* __restrict__ kout, uint32_t* __restrict__ vout, const int32_t* __restrict__ vrank, uint32_t vk) {
   constexpr int kSortItems = sort_items<K>();
   constexpr int kTileKeys = tile_keys<K>();
   __shared__ K skey[kTileKeys];
@@ -163,6 +163,7 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   // earlier lanes of its chunk with the same digit (8 ballots over the digit bits).  No
   // block barrier inside the run.  All the run's keys and values are loaded up front.
   const int wbase = wave * (kSortItems * 64);
+  const FastDiv vdiv(vk ? vk : 1u);
   K kreg[kSortItems];
   uint32_t vreg[kSortItems], lrank[kSortItems];
 #pragma unroll
@@ -171,6 +172,11 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const bool ok = jj < tile_n;
     kreg[i] = ok ? kin[base + jj] : K(0);
     vreg[i] = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
+    if (!vin && vrank && ok) {  // position n = r*vk + k becomes vrank[r]*vk + k
+      const uint32_t n = vreg[i];
+      const uint32_t r = vdiv.div(n);
+      vreg[i] = static_cast<uint32_t>(vrank[r]) * vk + (n - r * vk);
+    }
   }
   __syncthreads();  // loc, gpos and the zeroed counts are visible
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -232,7 +238,8 @@ size_t radix_sort_workspace(int64_t n, size_t key_bytes) {
 
 template <typename K>
 hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-                            int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st) {
+                            int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st, const int32_t* vrank,
+                            uint32_t vk) {
   if (n <= 0) return hipSuccess;
   constexpr int64_t kTileKeys = tile_keys<K>();
   const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
@@ -260,7 +267,7 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
                        ksrc, n, shift, tiles, counts, tile_counts);
     hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, st, counts, tiles, totals);
     hipLaunchKernelGGL((digit_scatter_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
-                       ksrc, vsrc, n, shift, tiles, counts, totals, tile_counts, kdst, vdst);
+                       ksrc, vsrc, n, shift, tiles, counts, totals, tile_counts, kdst, vdst, vrank, vk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ksrc = kdst;
@@ -270,8 +277,8 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
 }
 
 template hipError_t radix_sort_pairs<uint32_t>(const uint32_t*, uint32_t*, const uint32_t*, uint32_t*, int64_t,
-                                               int, int, void*, hipStream_t);
+                                               int, int, void*, hipStream_t, const int32_t*, uint32_t);
 template hipError_t radix_sort_pairs<uint64_t>(const uint64_t*, uint64_t*, const uint32_t*, uint32_t*, int64_t,
-                                               int, int, void*, hipStream_t);
+                                               int, int, void*, hipStream_t, const int32_t*, uint32_t);
 
 }  // namespace aknn

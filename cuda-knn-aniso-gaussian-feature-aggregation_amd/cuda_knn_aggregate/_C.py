@@ -516,9 +516,14 @@ def point_dots(rows, f, p, sigma, R, *, offsets=None):
 
 
 # ----------------------------------------------------------------------------- CSR
-def build_csr(idxs: torch.Tensor, P: int, key: str = "aniso"):
+def build_csr(idxs: torch.Tensor, P: int, key: str = "aniso", order=None):
     """Point-side transpose of ``idxs`` [B,Q,K]: ``(offsets int32 [B*P+1], entries int32
-    [B*Q*K])``, each point row's entries ``q*K+k`` ascending.  Deterministic."""
+    [B*Q*K])``, each point row's entries ``q*K+k`` ascending.  Deterministic.
+
+    With ``order`` (the cache's query order, int32 [B*Q]): entries number ``pos*K+k`` by
+    the query's position in that order, and the result is ``(offsets, entries, order)``;
+    the backward passes given it (with the same cache) read the query-side inputs in that
+    order."""
     _check_input(idxs, "idxs")
     if idxs.dtype != torch.int64 or idxs.dim() != 3:
         raise RuntimeError("idxs must be an int64 tensor [B, Q, K]")
@@ -531,9 +536,31 @@ def build_csr(idxs: torch.Tensor, P: int, key: str = "aniso"):
     ws_bytes = lib.aknn_csr_workspace_size(ctypes.byref(dims))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
+        if order is not None:
+            _lib.check(lib.aknn_build_csr_ordered(ctypes.byref(dims), idxs.data_ptr(), order.data_ptr(), off.data_ptr(),
+                                                  ent.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
+            return off, ent, order
         _lib.check(lib.aknn_build_csr(ctypes.byref(dims), idxs.data_ptr(), off.data_ptr(),
                                       ent.data_ptr(), ws.data_ptr(), ws_bytes, _stream(dev)))
     return off, ent
+
+
+def _cache_order(cache):
+    """The cache's query order (for an ordered CSR), or None."""
+    return cache[0] if cache is not None and len(cache) >= 1 and cache[0] is not None else None
+
+
+def _csr_parts(csr, cache):
+    """(offsets, entries, needs bit) of a CSR tuple; an ordered CSR must come with the
+    cache whose query order it was built in."""
+    if csr is None:
+        return None, None, 0
+    if len(csr) == 3:
+        order = _cache_order(cache)
+        if order is None or order.data_ptr() != csr[2].data_ptr():
+            raise RuntimeError("a CSR built in a query order needs the cache holding that order")
+        return csr[0], csr[1], _lib.CSR_QUERY_ORDER
+    return csr[0], csr[1], 0
 
 
 # ----------------------------------------------------------------------------- backward
@@ -632,7 +659,7 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
     slabs = _feature_slabs(F, dt)
     if slabs is not None:
         if mask & _lib.NEED_POINTS and csr is None:
-            csr = build_csr(k_idxs, P, key)
+            csr = build_csr(k_idxs, P, key, order=_cache_order(cache))
         df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
         res = None
         for s0, s1 in slabs:
@@ -660,10 +687,11 @@ def knn_aggregate_aniso_backward(grad_f_out, grad_w_out, q, p, f, sigma, R, w_ou
         "dR": torch.empty(B, P, 3, 3, dtype=dt, device=dev) if mask & _lib.NEED_DR else None,
     }
     if mask & _lib.NEED_POINTS and csr is None:
-        csr = build_csr(k_idxs, P, key)
+        csr = build_csr(k_idxs, P, key, order=_cache_order(cache))
     lib = _lib.load()
     dims = make_dims(B, Q, P, F, K, dt, key)
-    off, ent = csr if csr is not None else (None, None)
+    off, ent, ordered = _csr_parts(csr, cache)
+    mask |= ordered
     P_ = _lib.ptr
     if rows is not None:
         mask |= _lib.GRAD_BROADCAST
@@ -739,7 +767,7 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
     slabs = _feature_slabs(F, dt)
     if slabs is not None:
         if (mask & _lib.NEED_POINTS or (full and mask & _lib.NEED_DQ)) and csr is None:
-            csr = build_csr(k_idxs, P, key)
+            csr = build_csr(k_idxs, P, key, order=_cache_order(cache))
         dgf = torch.empty(B, Q, F, dtype=dt, device=q.device) if mask & _lib.NEED_DGRAD_F_OUT else None
         df = torch.empty(B, P, F, dtype=dt, device=q.device) if mask & _lib.NEED_DF else None
         res = None
@@ -774,10 +802,11 @@ def knn_aggregate_aniso_backward_2nd(grad_grad_q, grad_f_out, grad_w_out, q, p, 
         "dR": torch.empty(B, P, 3, 3, dtype=dt, device=dev) if mask & _lib.NEED_DR else None,
     }
     if (mask & _lib.NEED_POINTS or (full and mask & _lib.NEED_DQ)) and csr is None:
-        csr = build_csr(k_idxs, P, key)
+        csr = build_csr(k_idxs, P, key, order=_cache_order(cache))
     lib = _lib.load()
     dims = make_dims(B, Q, P, F, K, dt, key)
-    off, ent = csr if csr is not None else (None, None)
+    off, ent, ordered = _csr_parts(csr, cache)
+    mask |= ordered
     P_ = _lib.ptr
     if rows is not None:
         mask |= _lib.GRAD_BROADCAST

@@ -222,7 +222,8 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
             needs = (True,) * 5
         P = p.shape[1]
         has_q = q.shape[0] * q.shape[1] > 0
-        csr = _C.build_csr(idxs, P, key) if _point_needs(needs[1:]) and has_q else None
+        csr = (_C.build_csr(idxs, P, key, order=_C._cache_order(cache)) if _point_needs(needs[1:]) and has_q
+               else None)
         dq, dp, df, dsigma, dR = _C.knn_aggregate_aniso_backward(
             grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs, key=key, csr=csr, needs=needs,
             cache=cache, check=False, use_grad_w_out=use_w)
@@ -233,7 +234,7 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         if cache is not None:
             saved += list(cache)
         ctx.save_for_backward(*saved)
-        ctx.has_csr = csr is not None
+        ctx.csr_n = 0 if csr is None else len(csr)
         ctx.has_cache = cache is not None
         ctx.key = key
         # outputs nothing depends on arrive as None (not zeros): the second order then
@@ -246,7 +247,7 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         saved = ctx.saved_tensors
         grad_f_out, grad_w_out, q, p, f, sigma, R, w_out, idxs = saved[:9]
         rest = list(saved[9:])
-        csr = (rest.pop(0), rest.pop(0)) if ctx.has_csr else None
+        csr = tuple(rest.pop(0) for _ in range(ctx.csr_n)) if ctx.csr_n else None
         cache = tuple(rest[:3]) if ctx.has_cache else None
         ni = ctx.needs_input_grad
         # outputs: d grad_f_out, dq, dp, df, dsigma, dR (and d grad_w_out on the grad_w_out path)
@@ -258,7 +259,7 @@ class KNNAnisotropicAggregateBackward(torch.autograd.Function):
         if (grad_grad_q is None and not full) or not (any(needs) or need_dgw):
             return (None,) * 13
         if csr is None and (_point_needs(needs[2:]) or (full and needs[1])) and q.shape[0] * q.shape[1] > 0:
-            csr = _C.build_csr(idxs, p.shape[1], ctx.key)
+            csr = _C.build_csr(idxs, p.shape[1], ctx.key, order=_C._cache_order(cache))
         res = _C.knn_aggregate_aniso_backward_2nd(
             None if grad_grad_q is None else grad_grad_q.contiguous(),
             _C.upstream(grad_f_out),

@@ -37,6 +37,7 @@ NEED_DGRAD_F_OUT = 32
 SECOND_FULL = 64
 GRAD_BROADCAST = 128  # grad_f_out given as one row per cloud ([B, F])
 NEED_DGRAD_W_OUT = 256  # aknn_backward2_w: d grad_w_out
+CSR_QUERY_ORDER = 512  # the CSR numbers queries by their position in the cache's query order
 NEED_POINTS = NEED_DP | NEED_DF | NEED_DSIGMA | NEED_DR
 
 EXPORTS = (
@@ -50,6 +51,7 @@ EXPORTS = (
     "aknn_aggregate",
     "aknn_csr_workspace_size",
     "aknn_build_csr",
+    "aknn_build_csr_ordered",
     "aknn_backward_workspace_size",
     "aknn_backward",
     "aknn_backward2",
@@ -134,6 +136,7 @@ _SIGS = {
     "aknn_aggregate": (ctypes.c_int, [_DIMS] + [_P] * 6 + [_P] * 3 + [_P]),
     "aknn_csr_workspace_size": (ctypes.c_size_t, [_DIMS]),
     "aknn_build_csr": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, ctypes.c_size_t, _P]),
+    "aknn_build_csr_ordered": (ctypes.c_int, [_DIMS, _P, _P, _P, _P, _P, ctypes.c_size_t, _P]),
     "aknn_backward_workspace_size": (ctypes.c_size_t, [_DIMS, ctypes.c_uint32]),
     "aknn_backward": (ctypes.c_int, [_DIMS, ctypes.c_uint32] + [_P] * 9 + [_CACHE, _P, ctypes.c_size_t]
                       + [_P] * 5 + [_P]),

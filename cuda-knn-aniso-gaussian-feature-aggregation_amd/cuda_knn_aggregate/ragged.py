@@ -115,7 +115,8 @@ def forward(layout, q, p, f, sigma, R, K, *, key="aniso", tune=None, cache=None)
     return [f_out, w_out, dists, idxs]
 
 
-def build_csr(layout, idxs, K):
+def build_csr(layout, idxs, K, order=None):
+    """CSR of a ragged batch (``_C.build_csr``; with ``order`` in that query order)."""
     dev = idxs.device
     off = torch.empty(layout.P + 1, dtype=torch.int32, device=dev)
     ent = torch.empty(max(layout.Q * K, 1), dtype=torch.int32, device=dev)
@@ -124,6 +125,11 @@ def build_csr(layout, idxs, K):
     ws_bytes = lib.aknn_csr_workspace_size(ctypes.byref(dims))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
+        if order is not None:
+            _lib.check(lib.aknn_build_csr_ordered(ctypes.byref(dims), idxs.data_ptr(), order.data_ptr(),
+                                                  off.data_ptr(), ent.data_ptr(), ws.data_ptr(), ws_bytes,
+                                                  _C._stream(dev)))
+            return off, ent, order
         _lib.check(lib.aknn_build_csr(ctypes.byref(dims), idxs.data_ptr(), off.data_ptr(), ent.data_ptr(),
                                       ws.data_ptr(), ws_bytes, _C._stream(dev)))
     return off, ent
@@ -163,7 +169,7 @@ def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, nee
     slabs = _C._feature_slabs(F, dt)
     if slabs is not None:  # column ranges: column outputs side by side, the rest summed in order
         if need_csr and csr is None:
-            csr = build_csr(layout, idxs, K)
+            csr = build_csr(layout, idxs, K, order=_C._cache_order(cache))
         cols = {n: (torch.empty(layout.Q if n == "dgf" else layout.P, F, dtype=dt, device=q.device)
                     if mask & _C._NEED_BITS[n] else None) for n in ("dgf", "df") if n in names}
         res = None
@@ -185,10 +191,11 @@ def _backward_impl(layout, second, gg, g, q, p, f, sigma, R, idxs, key, csr, nee
         return res
     outs = {n: (torch.empty(shapes[n], dtype=dt, device=dev) if mask & _C._NEED_BITS[n] else None) for n in names}
     if need_csr and csr is None:
-        csr = build_csr(layout, idxs, K)
+        csr = build_csr(layout, idxs, K, order=_C._cache_order(cache))
     lib = _lib.load()
     dims = layout.dims(F, K, dt, key)
-    off, ent = csr if csr is not None else (None, None)
+    off, ent, ordered = _C._csr_parts(csr, cache)
+    mask |= ordered
     P_ = _lib.ptr
     rows = None if full else _rows(layout, g)
     if rows is not None:
@@ -271,11 +278,12 @@ class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, grad_f_out, q, p, f, sigma, R, idxs, layout, key, needs, cache):
-        csr = build_csr(layout, idxs, int(idxs.shape[1])) if any(needs[1:]) and layout.Q > 0 else None
+        csr = (build_csr(layout, idxs, int(idxs.shape[1]), order=_C._cache_order(cache))
+               if any(needs[1:]) and layout.Q > 0 else None)
         out = backward(layout, grad_f_out, q, p, f, sigma, R, idxs, key=key, csr=csr, needs=needs, cache=cache)
         saved = [grad_f_out, q, p, f, sigma, R, idxs] + (list(csr) if csr else []) + (list(cache) if cache else [])
         ctx.save_for_backward(*saved)
-        ctx.layout, ctx.key, ctx.has_csr, ctx.has_cache = layout, key, csr is not None, cache is not None
+        ctx.layout, ctx.key, ctx.csr_n, ctx.has_cache = layout, key, 0 if csr is None else len(csr), cache is not None
         # outputs the loss does not use arrive as None, not as zeros that would select the
         # full second order
         ctx.set_materialize_grads(False)
@@ -286,7 +294,7 @@ class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
         saved = list(ctx.saved_tensors)
         grad_f_out, q, p, f, sigma, R, idxs = saved[:7]
         rest = saved[7:]
-        csr = (rest.pop(0), rest.pop(0)) if ctx.has_csr else None
+        csr = tuple(rest.pop(0) for _ in range(ctx.csr_n)) if ctx.csr_n else None
         cache = tuple(rest[:3]) if ctx.has_cache else None
         ni = ctx.needs_input_grad
         needs = (ni[0], ni[1], ni[2], ni[3], ni[4], ni[5])
@@ -297,7 +305,7 @@ class KNNAnisotropicAggregateRaggedBackward(torch.autograd.Function):
         if (gg_q is None and not full) or not any(needs):
             return (None,) * 11
         if csr is None and (any(needs[2:]) or (full and needs[1])) and ctx.layout.Q > 0:
-            csr = build_csr(ctx.layout, idxs, int(idxs.shape[1]))
+            csr = build_csr(ctx.layout, idxs, int(idxs.shape[1]), order=_C._cache_order(cache))
         out = backward2(ctx.layout, gg_q, grad_f_out, q, p, f, sigma, R, idxs, key=ctx.key, csr=csr,
                         needs=needs, cache=cache, gg_p=ups[0], gg_f=ups[1], gg_sigma=ups[2], gg_R=ups[3])
         return tuple(out) + (None,) * 5

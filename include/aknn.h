@@ -319,12 +319,25 @@ AKNN_API int aknn_iso_forward(const aknn_dims* dims, const aknn_tune* tune, cons
 AKNN_API size_t aknn_csr_workspace_size(const aknn_dims* dims);
 AKNN_API int aknn_build_csr(const aknn_dims* dims, const int64_t* idxs, int32_t* csr_offsets,
                    int32_t* csr_entries, void* workspace, size_t workspace_bytes, void* stream);
+/* The same transpose in a spatial numbering of the queries (no counterpart in the
+ * reference): query_order [query rows] lists the query rows in the forward's spatial
+ * order (aknn_cache.query_order); entry (q, k) is numbered (position of q in that
+ * order)*K + k, each row's entries still in ascending q*K+k order (so the backward
+ * sums, and their bits, are those of the plain transpose).  Given AKNN_CSR_QUERY_ORDER
+ * in needs and the same cache, the backward passes read the query-side inputs from
+ * copies in that order, where the queries of one point row lie close together.  Same
+ * workspace as aknn_build_csr. */
+#define AKNN_CSR_QUERY_ORDER 512u
+AKNN_API int aknn_build_csr_ordered(const aknn_dims* dims, const int64_t* idxs, const int32_t* query_order,
+                                    int32_t* csr_offsets, int32_t* csr_entries, void* workspace,
+                                    size_t workspace_bytes, void* stream);
 
 /* Device workspace of aknn_backward / aknn_backward2 for a `needs` mask (0 when none is
  * needed): with dq and point gradients both requested, the point kernels hand each
  * (query, slot) entry's dq share to a per-query sum instead of a second gather.  With
  * AKNN_GRAD_BROADCAST and dq requested: the dot records (dtype[point rows][16]),
- * unless the cache brings them. */
+ * unless the cache brings them.  With AKNN_CSR_QUERY_ORDER: also the ordered copies of
+ * the query-side inputs the point kernels read. */
 AKNN_API size_t aknn_backward_workspace_size(const aknn_dims* dims, uint32_t needs);
 
 /* Dot records for a broadcast upstream (no counterpart in the reference): for every
