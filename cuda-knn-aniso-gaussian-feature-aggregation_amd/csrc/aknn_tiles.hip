@@ -30,6 +30,7 @@
 #include "aknn_bounds.h"
 #include "aknn_common.h"
 #include "aknn_internal.h"
+#include "aknn_order.h"
 
 namespace aknn {
 
@@ -375,41 +376,60 @@ struct TopList {
   __device__ __forceinline__ int id(int s) const { return v[s]; }
 };
 
-// FP32 form: one 64-bit word per entry, the order-preserving encoding of the key above
-// the (non-negative) index, so that unsigned 64-bit order is (key, index) order and an
-// insertion step is one 64-bit compare and two 64-bit selects, in place.  -0 is encoded as +0 (they
-// compare equal as keys); a -0 key is restored on output by the caller.  NaN keys are
-// never inserted.  `lastk` caches the decoded last key for the pruning tests.
+// FP64 min / max of two list words (aknn_order.h: finite doubles, never NaN), written
+// out so that the compiler adds no NaN-quieting canonicalisation of their operands.
+__device__ __forceinline__ double word_max(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double word_min(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// FP32 form: one word per entry, (key, index) encoded as a finite double whose numeric
+// order is (key, index) order (aknn_order.h), so that an insertion is
+// w[i] = min(w[i], max(w[i-1], x)) on the old words: two FP64 min/max per slot, in
+// place.  -0 is encoded as +0 (they compare equal as keys); a -0 key is restored on
+// output by the caller.  NaN keys are never inserted.  `lastk` caches the decoded last
+// key for the pruning tests.
 template <int N>
 struct TopList<float, N> {
-  uint64_t w[N];
+  double w[N];
   float lastk;
-  static __device__ __forceinline__ uint64_t pack(float c, int ci) {
-    return (static_cast<uint64_t>(OrdKey<float>::enc(c + 0.0f)) << 32) | static_cast<uint32_t>(ci);
+  static __device__ __forceinline__ double word(float c, int ci) {
+    return __longlong_as_double(static_cast<long long>(list_word(c, ci)));
   }
   __device__ __forceinline__ void init() {
 #pragma unroll
-    for (int s = 0; s < N; ++s) w[s] = pack(__builtin_inff(), kNoPoint);
+    for (int s = 0; s < N; ++s) w[s] = word(__builtin_inff(), kNoPoint);
     lastk = __builtin_inff();
   }
   __device__ __forceinline__ float last() const { return lastk; }
   __device__ __forceinline__ void insert(float c, int ci) {
-    const uint64_t x = pack(c, ci);
-    // shift insertion, last slot first: slot i takes slot i-1 when x belongs before
-    // it, x when x lands at i, else keeps its word (the words are distinct and sorted,
-    // so the compares are monotone in i)
+    const double x = word(c, ci);
+    // slots from the last down, two at a time (independent pairs, so that no result is
+    // read by the next instruction); each reads w[i-1] before it is updated
 #pragma unroll
-    for (int i = N - 1; i >= 0; --i) {
-      const bool here = x < w[i];
-      const bool before = i > 0 && x < w[i > 0 ? i - 1 : 0];
-      w[i] = before ? w[i > 0 ? i - 1 : 0] : (here ? x : w[i]);
+    for (int i = N - 1; i >= 2; i -= 2) {
+      const double ta = word_max(w[i - 1], x);
+      const double tb = word_max(w[i - 2], x);
+      w[i] = word_min(w[i], ta);
+      w[i - 1] = word_min(w[i - 1], tb);
     }
+    // left: slots 1 and 0 (N even) or slot 0 (N odd)
+    if constexpr (N % 2 == 0) w[1] = word_min(w[1], word_max(w[0], x));
+    w[0] = word_min(w[0], x);
     lastk = key(N - 1);
   }
   __device__ __forceinline__ float key(int s) const {
-    return OrdKey<float>::dec(static_cast<uint32_t>(w[s] >> 32));
+    return list_word_key(static_cast<uint64_t>(__double_as_longlong(w[s])));
   }
-  __device__ __forceinline__ int id(int s) const { return static_cast<int>(static_cast<uint32_t>(w[s])); }
+  __device__ __forceinline__ int id(int s) const {
+    return list_word_index(static_cast<uint64_t>(__double_as_longlong(w[s])));
+  }
 };
 
 // One 16-value point record, read with wave-uniform addresses (scalar loads).
