@@ -112,12 +112,22 @@ AKNN_HD inline float stored_mu(double mu, bool bad) {
 
 // mu * dist^2(q, box), in the key's precision, for the box [lo, hi] (float corners that
 // contain the exact coordinates); meaningful for mu >= 0.
+// The offset along one axis is q - clamp(q, lo, hi): -(lo - q) below the box, q - hi
+// above it (the same rounded magnitudes as lo - q and q - hi), 0 inside.  On the device
+// the FP32 clamp is one median-of-three (the same value for lo <= hi).
+template <typename T>
+AKNN_HD inline T clamp_to_box(T q, float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_fmed3f(q, lo, hi);
+#endif
+  return fmin(fmax(q, static_cast<T>(lo)), static_cast<T>(hi));
+}
 template <typename T>
 AKNN_HD inline T box_lower_bound(float lx, float ly, float lz, float hx, float hy, float hz, float mu, T qx, T qy,
                                  T qz) {
-  const T dx = fmax(fmax(static_cast<T>(lx) - qx, qx - static_cast<T>(hx)), T(0));
-  const T dy = fmax(fmax(static_cast<T>(ly) - qy, qy - static_cast<T>(hy)), T(0));
-  const T dz = fmax(fmax(static_cast<T>(lz) - qz, qz - static_cast<T>(hz)), T(0));
+  const T dx = qx - clamp_to_box<T>(qx, lx, hx);
+  const T dy = qy - clamp_to_box<T>(qy, ly, hy);
+  const T dz = qz - clamp_to_box<T>(qz, lz, hz);
   return static_cast<T>(mu) * (dx * dx + dy * dy + dz * dz);
 }
 

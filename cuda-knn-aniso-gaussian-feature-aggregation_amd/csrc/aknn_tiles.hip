@@ -473,6 +473,30 @@ __device__ __forceinline__ int rec_index(T v) {
   else return static_cast<int>(__double_as_longlong(v));
 }
 
+// A box header {lo x y z, hi x y z, mu, -} read with a wave-uniform address (scalar
+// loads), and whether any lane still needs the box: mu * dist^2(q, box) not above the
+// lane's K-th key thr (box_needed; lanes without a query hold thr = -inf), decided on the
+// wave's ballot.  A negative mu ("always scan") is tested on the scalar unit: such a box
+// is needed by the wave (its first lane is always a query).
+struct BoxHdr {
+  float v[8];
+};
+__device__ __forceinline__ BoxHdr load_hdr(const float* __restrict__ h) {
+  BoxHdr o;
+  const float4* h4 = reinterpret_cast<const float4*>(h);
+  const float4 a = h4[0], b = h4[1];
+  o.v[0] = a.x; o.v[1] = a.y; o.v[2] = a.z; o.v[3] = a.w;
+  o.v[4] = b.x; o.v[5] = b.y; o.v[6] = b.z; o.v[7] = b.w;
+  return o;
+}
+template <typename T>
+__device__ __forceinline__ bool box_any(const BoxHdr& h, T qx, T qy, T qz, T thr) {
+  // branch-free, so that the loads of a batch of headers stay together
+  const bool always = __float_as_int(h.v[6]) < 0;
+  const T lb = box_lower_bound<T>(h.v[0], h.v[1], h.v[2], h.v[3], h.v[4], h.v[5], h.v[6], qx, qy, qz);
+  return always | (__builtin_amdgcn_ballot_w64(!(lb > thr)) != 0);
+}
+
 // Outward walk over [lo, hi) from c: c, c+1, c-1, c+2, c-2, ... (each index once).
 struct Outward {
   int up, down, lo, hi;
@@ -556,7 +580,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   const int64_t pos = q0 + lane;
   const bool valid = pos < qend;
   int64_t row = 0;
-  T qx = __builtin_nan(""), qy = qx, qz = qx;
+  // lanes past the cloud's queries keep a finite dummy query and a bound of -inf (thr
+  // below): they need no box and insert nothing
+  T qx = T(0), qy = qx, qz = qx;
   if (valid) {
     row = qperm[pos];
     qx = q[row * 3 + 0];
@@ -600,7 +626,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   // Every way's list is a subset's top-KMAX, so its last key bounds the full top-KMAX
   // from above: a tile skipped against thr holds no point of the final result, whatever
   // the timing of the exchange.
-  T thr = (ways > 1 && valid) ? Ord::dec(bnd[pos]) : __builtin_inf();
+  T thr = valid ? (ways > 1 ? Ord::dec(bnd[pos]) : static_cast<T>(__builtin_inf())) : static_cast<T>(-__builtin_inf());
   unsigned n_super = 0, n_tile = 0, n_scan = 0, n_pts = 0;
   Outward so(sb, se, seed_s);
   for (int sstep = 0; sstep < se - sb; ++sstep) {
@@ -608,7 +634,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     ++n_super;
     if (ways > 1 && valid)
       thr = fmin(thr, Ord::dec(__hip_atomic_load(bnd + pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-    if (!__any(box_needed<T>(shdr + static_cast<int64_t>(s) * 8, qx, qy, qz, fmin(L.last(), thr), valid)))
+    if (!box_any<T>(load_hdr(shdr + static_cast<int64_t>(s) * 8), qx, qy, qz, fmin(L.last(), thr)))
       continue;
     const int t0 = tb + (s - sb) * super;
     const int t1 = min(te, t0 + super);
@@ -617,7 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
       const int t = to.next(tstep);
       if ((t - tb) % ways != iway) continue;
       ++n_tile;
-      if (!__any(box_needed<T>(hdr + static_cast<int64_t>(t) * 8, qx, qy, qz, fmin(L.last(), thr), valid)))
+      if (!box_any<T>(load_hdr(hdr + static_cast<int64_t>(t) * 8), qx, qy, qz, fmin(L.last(), thr)))
         continue;
       ++n_scan;
       const int cnt = tcnt[t];
@@ -626,10 +652,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
       // lim = fmin(last kept key, thr): thr is fixed within a tile, so lim changes only
       // with an insertion
       T lim = fmin(L.last(), thr);
+      // headers four at a time (their scalar loads issued together, one wait); sub-tiles
+      // past the tile's records (whole padding) are masked off
+      constexpr int kBatch = kSubs < 4 ? kSubs : 4;
       unsigned m = 0;
 #pragma unroll
-      for (int u = 0; u < kSubs; ++u)
-        if ((SUBREC * u < cnt) && __any(box_needed<T>(sh + u * 8, qx, qy, qz, lim, valid))) m |= 1u << u;
+      for (int u0 = 0; u0 < kSubs; u0 += kBatch) {
+        BoxHdr hb[kBatch];
+#pragma unroll
+        for (int v = 0; v < kBatch; ++v) hb[v] = load_hdr(sh + (u0 + v) * 8);
+#pragma unroll
+        for (int v = 0; v < kBatch; ++v)
+          if (box_any<T>(hb[v], qx, qy, qz, lim)) m |= 1u << (u0 + v);
+      }
+      const int live = (cnt + SUBREC - 1) / SUBREC;
+      if (live < 32) m &= (1u << live) - 1u;
       if (m == 0) continue;
       const int total = SUBREC * __builtin_popcount(m);
       n_pts += total;
