@@ -559,9 +559,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     return -1;
   };
   auto walk = [&](const int64_t lin0) __attribute__((always_inline)) {
+  // item numbers, query tiles and ways fit in 32 bits (the host checks the counts)
   int64_t b, q0;
-  const int64_t way = lin0 % ways;
-  const int64_t lin = lin0 / ways;
+  const uint32_t item = static_cast<uint32_t>(lin0);
+  const uint32_t way = ways == 1 ? 0u : item % static_cast<uint32_t>(ways);
+  const uint32_t lin = ways == 1 ? item : item / static_cast<uint32_t>(ways);
   if (qtiles) {
     const int64_t t = lin;
     int64_t lo = 0, hi = cl.B - 1;
@@ -572,9 +574,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
     b = lo;
     q0 = cl.q_begin(b) + (t - qtiles[b]) * 64;
   } else {
-    const int64_t per = (cl.Q + 63) / 64;  // query tiles per cloud
-    b = lin / per;
-    q0 = cl.q_begin(b) + (lin % per) * 64;
+    const uint32_t per = static_cast<uint32_t>((cl.Q + 63) / 64);  // query tiles per cloud
+    const uint32_t bb = cl.B == 1 ? 0u : lin / per;
+    b = bb;
+    q0 = cl.q_begin(b) + static_cast<int64_t>(lin - bb * per) * 64;
   }
   const int64_t qend = cl.q_end(b);
   const int64_t pos = q0 + lane;
