@@ -574,6 +574,15 @@ def main():
                         "tiles_scanned_per_wave": c[2] / c[4], "points_scanned_per_query": c[3] / c[4]}
     else:
         search_stats = None
+    # degrees of the CSR point rows the point kernels walk (one wave per row): the skew
+    # a heavy row puts on their tail
+    _, idxs0 = _C.knn_aniso(b.q.detach(), b.p.detach(), b.sigma.detach(), b.R.detach(), K, tune=tune)
+    off = _C.build_csr(idxs0, P)[0].long()
+    deg = (off[1:] - off[:-1]).float()
+    csr_degree = {"mean": float(deg.mean()), "max": int(deg.max()), "p99": float(torch.quantile(deg, 0.99)),
+                  "p999": float(torch.quantile(deg, 0.999)), "empty_rows": int((deg == 0).sum()),
+                  "entries_in_rows_over_4x_mean": float(deg[deg > 4 * deg.mean()].sum() / deg.sum())}
+    del idxs0, off, deg
 
     traffic, traffic_src = measured_traffic(args.workload)
 
@@ -643,6 +652,7 @@ def main():
         "roofline_gather": roof_agg,
         "kernels": kernels,
         "search_stats": search_stats,
+        "csr_degree": csr_degree,
     }
     if not args.no_extras:
         # the step's gradients as one captured HIP graph (no autograd dispatch between launches)
