@@ -76,8 +76,8 @@ __device__ __forceinline__ T group_sum(T v) {
 // Broadcast upstream (one grad_f_out row per cloud, BC): this lane's slice of the
 // point row's cloud row, and a = g . f[row] reduced across the row in the association
 // of the per-entry dot products (row_transpose_sum over 16 equal partials: the same
-// pairing tree as for any one entry), so a and every df term keep the bits of a
-// materialised upstream.
+// pairing tree as for any one entry), so a, and with it dp, dsigma and dR, keep the
+// bits of a materialised upstream.
 template <typename T, int VEC, int CH>
 __device__ __forceinline__ T bcast_row_dot(const T* __restrict__ g, int64_t b, int64_t F, const int64_t (&fofs)[CH],
                                            const bool (&fok)[CH], const T (&fsl)[CH][VEC], T (&gbc)[CH][VEC],
@@ -159,6 +159,9 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
   T gbc[BC ? CH : 1][VEC];
   T a_row = T(0);
   if constexpr (BC) a_row = bcast_row_dot<T, VEC, CH>(g, L.cl.cloud_of_point(row), F, fofs, fok, fsl, gbc, r);
+  // broadcast upstream: every entry's df term is coef * (the cloud's row), so df is the
+  // row times the sum of the coefficients (each lane sums its own entries, in order)
+  T csum = T(0);
 
   for (int eb = e0; eb < e1; eb += 16 * G) {
     const int e = eb + 16 * gq + r;
@@ -184,20 +187,14 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
     }
     const int qi32 = static_cast<int>(qrow);
 
-    T cj[16];
-    pstatic_for<0, 16>([&](auto J) { cj[J.value] = prow_get<J.value>(coef); });
     T a;
     if constexpr (BC) {
-      // every entry's upstream row is the cloud's row, already in registers
-      pstatic_for<0, 16>([&](auto J) {
-#pragma unroll
-        for (int c = 0; c < CH; ++c)
-#pragma unroll
-          for (int i = 0; i < VEC; ++i) dfa[c][i] += cj[J.value] * gbc[c][i];
-      });
+      csum += coef;  // zero for lanes without an entry (w = 0)
       a = a_row;
       (void)qi32;
     } else {
+      T cj[16];
+      pstatic_for<0, 16>([&](auto J) { cj[J.value] = prow_get<J.value>(coef); });
       // All 16 row gathers of the batch are issued back to back: no per-lane branch in
       // this loop (lanes past F read a clamped in-row address and mask the value).
       int qj[16];
@@ -282,7 +279,13 @@ __global__ __launch_bounds__(256) void point_kernel(PointLaunch L) {
 
 #pragma unroll
   for (int i = 0; i < 15; ++i) acc[i] = group_sum<G>(row_sum(acc[i]));
-  if constexpr (G > 1) {
+  if constexpr (BC) {
+    const T tot = group_sum<G>(row_sum(csum));
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) dfa[c][i] = tot * gbc[c][i];
+  } else if constexpr (G > 1) {
 #pragma unroll
     for (int c = 0; c < CH; ++c)
 #pragma unroll
