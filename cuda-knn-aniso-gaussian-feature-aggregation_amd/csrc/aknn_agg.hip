@@ -231,7 +231,9 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
       for (int c = 0; c < CH; ++c) {
 #pragma unroll
         for (int i = 0; i < VEC; ++i) {
-          const T x = fok[c] ? fv[J.value][c].v[i] : T(0);
+          // lanes past F read a clamped in-row element: the forward modes never store
+          // their accumulators, the backward modes mask it out of the dot product
+          const T x = kBack ? (fok[c] ? fv[J.value][c].v[i] : T(0)) : fv[J.value][c].v[i];
           if constexpr (MODE != kBwd1) acc[c][i] += cj[J.value] * x;
           if constexpr (kBack) dot += gsl[c][i] * x;
         }
@@ -398,8 +400,8 @@ hipError_t launch_query_grad_full(const QueryLaunch& L, hipStream_t st) { return
 // the dot records (L.dots: {p, R, s, a} per point row), so dq needs no feature rows at
 // all -- per query only its idxs row, the K dot records (L2-resident) and, in the second
 // order, grad_grad_q.  One 16-lane row per query, lane r on slot kb + r.  Each slot's
-// share is the point kernels' per-entry share expression, and the shares are added in
-// slot order starting from zero, as contrib_reduce adds them.
+// share is the point kernels' per-entry share expression; the 16 shares of a batch are
+// summed by a fixed DPP tree over the row (deterministic), batches in slot order.
 //
 // A lane-per-record gather would touch a different cache line in every lane of every
 // load and keep the texture addresser busy for the whole kernel.  Instead each wave
@@ -495,15 +497,11 @@ __global__ __launch_bounds__(256) void dq_dots_kernel(QueryLaunch L) {
       c1 = T(2) * aw * t1;
       c2 = T(2) * aw * t2;
     }
-    const int nk = K - kb;  // slots in this batch (uniform)
-    static_for<0, 16>([&](auto J) {
-      const T x0 = row_get<J.value>(c0), x1 = row_get<J.value>(c1), x2 = row_get<J.value>(c2);
-      if (J.value < nk) {
-        s0 += x0;
-        s1 += x1;
-        s2 += x2;
-      }
-    });
+    // the batch's slot shares summed over the row by a fixed DPP tree (slots past K add
+    // zero), batches in order
+    s0 += row_sum(have ? c0 : T(0));
+    s1 += row_sum(have ? c1 : T(0));
+    s2 += row_sum(have ? c2 : T(0));
   }
   if (row_ok && r < 3) static_cast<T*>(L.dq)[row * 3 + r] = r == 0 ? s0 : (r == 1 ? s1 : s2);
 }
