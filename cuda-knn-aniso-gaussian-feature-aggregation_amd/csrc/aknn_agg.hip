@@ -115,11 +115,14 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
   T gsl[CH][VEC];
   int64_t fofs[CH];  // clamped into the row for lanes past F
   bool fok[CH];
+  const char* fbase[CH];  // this lane's slice in feature row 0 (a pointer: stays a global load)
+  const uint64_t frow_bytes = static_cast<uint32_t>(F * static_cast<int64_t>(sizeof(T)));
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int64_t ft = static_cast<int64_t>(c * 16 + r) * VEC;
     fok[c] = ft < F;
     fofs[c] = fok[c] ? ft : 0;
+    fbase[c] = reinterpret_cast<const char*>(f + fofs[c]);
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c)
@@ -219,9 +222,12 @@ __global__ __launch_bounds__(256) void query_kernel(QueryLaunch L) {
     });
     VecT<T, VEC> fv[16][CH];
     static_for<0, 16>([&](auto J) {
-      const T* __restrict__ frow = f + (pb + nj[J.value]) * F;
+      // point rows < 2^31 and feature rows < 4 GB (checked by the C ABI): one 32 x 32 -> 64-bit
+      // multiply-add per row address
+      const uint64_t prow = static_cast<uint32_t>(pb + nj[J.value]);
 #pragma unroll
-      for (int c = 0; c < CH; ++c) fv[J.value][c] = *reinterpret_cast<const VecT<T, VEC>*>(frow + fofs[c]);
+      for (int c = 0; c < CH; ++c)
+        fv[J.value][c] = *reinterpret_cast<const VecT<T, VEC>*>(fbase[c] + prow * frow_bytes);
     });
     if constexpr (MODE == kIso) static_for<0, 16>([&](auto J) { wsum += cj[J.value]; });
     T part[16];
