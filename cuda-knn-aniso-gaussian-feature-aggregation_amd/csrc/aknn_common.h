@@ -26,9 +26,18 @@ struct Clouds {
   const int64_t* q_off;
   const int64_t* p_off;
   int64_t B, Q, P;  // dense per-cloud sizes, or totals for ragged batches
+  // The row arguments below are rows of the batch: [0, B*Q) queries, [0, B*P) points.
 
+  // Cloud of a dense row (row < B * n): no division for one cloud, a 32-bit one when the
+  // operands fit (the 64-bit division is a long scalar and vector sequence).
+  __device__ __forceinline__ int64_t dense_cloud(int64_t row, int64_t n) const {
+    if (B == 1) return 0;
+    if (((static_cast<uint64_t>(row) | static_cast<uint64_t>(n)) >> 32) == 0)
+      return static_cast<int64_t>(static_cast<uint32_t>(row) / static_cast<uint32_t>(n));
+    return row / n;
+  }
   __device__ __forceinline__ int64_t cloud_of_query(int64_t row) const {
-    if (!q_off) return row / Q;
+    if (!q_off) return dense_cloud(row, Q);
     int64_t lo = 0, hi = B - 1;  // last b with q_off[b] <= row
     while (lo < hi) {
       int64_t mid = (lo + hi + 1) >> 1;
@@ -37,7 +46,7 @@ struct Clouds {
     return lo;
   }
   __device__ __forceinline__ int64_t cloud_of_point(int64_t row) const {
-    if (!p_off) return row / P;
+    if (!p_off) return dense_cloud(row, P);
     int64_t lo = 0, hi = B - 1;  // last b with p_off[b] <= row
     while (lo < hi) {
       int64_t mid = (lo + hi + 1) >> 1;

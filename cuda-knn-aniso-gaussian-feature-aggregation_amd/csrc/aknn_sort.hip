@@ -179,21 +179,28 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     }
   }
   __syncthreads();  // loc, gpos and the zeroed counts are visible
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
     const bool ok = wbase + i * 64 + lane < tile_n;
     const uint32_t d = ok ? static_cast<uint32_t>((kreg[i] >> shift) & 255u) : 256u;
-    unsigned long long same = __ballot(ok);
+    // lanes with the same digit: start from the lanes holding a key, and for each digit
+    // bit keep the lanes whose bit agrees with this lane's (the ballot of the bit, or
+    // its complement); bm is the lane's bit as an all-ones or all-zeros word, so each
+    // half of the mask takes one three-input bit operation per bit
+    const uint64_t okm = __builtin_amdgcn_ballot_w64(ok);
+    uint32_t slo = static_cast<uint32_t>(okm), shi = static_cast<uint32_t>(okm >> 32);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const unsigned long long m = __ballot(ok && ((d >> b) & 1u));
-      same &= ((d >> b) & 1u) ? m : ~m;
+      const uint32_t bm = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+      const uint64_t m = __builtin_amdgcn_ballot_w64(bm != 0u);
+      slo &= ~(static_cast<uint32_t>(m) ^ bm);
+      shi &= ~(static_cast<uint32_t>(m >> 32) ^ bm);
     }
-    const uint32_t rank_in = static_cast<uint32_t>(__popcll(same & lt));
+    // rank among the earlier lanes of the chunk with the same digit, and their count
+    const uint32_t rank_in = __builtin_amdgcn_mbcnt_hi(shi, __builtin_amdgcn_mbcnt_lo(slo, 0u));
     const uint32_t prev = ok ? wcnt[wave][d] : 0u;
     lrank[i] = prev + rank_in;
-    if (ok && rank_in == 0) wcnt[wave][d] = prev + static_cast<uint32_t>(__popcll(same));
+    if (ok && rank_in == 0) wcnt[wave][d] = prev + static_cast<uint32_t>(__popc(slo) + __popc(shi));
     __builtin_amdgcn_wave_barrier();  // the next chunk's reads follow this chunk's update
   }
   __syncthreads();
