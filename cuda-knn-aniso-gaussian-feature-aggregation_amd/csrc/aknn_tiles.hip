@@ -54,6 +54,7 @@ struct PrunedState {
   int32_t* lid;     //                                    and indices
   void* bnd;        // split walk: shared per-query bound (ordered encoding) [qrows]
   unsigned* work;   // persistent search: next-item counters, one per XCD [kXcds]
+  int32_t* torder;  // query tiles in walk order (widest first within each XCD's range) [nqtiles]
   size_t total;
 };
 
@@ -522,7 +523,7 @@ template <typename T, int KMAX, int KEY, int SUBREC, int WAVES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVES : search_waves<T, KMAX>(),
                                                                    WAVES ? WAVES : search_waves<T, KMAX>()))) void knn_pruned_kernel(
     const T* __restrict__ q, Clouds cl, const int64_t* __restrict__ qtiles,
-    const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys,
+    const uint32_t* __restrict__ qperm, const uint64_t* __restrict__ qkeys, const int32_t* __restrict__ torder,
     const int64_t* __restrict__ ptiles, const int64_t* __restrict__ stiles,
     const T* __restrict__ rec, const float* __restrict__ hdr, const float* __restrict__ shdr,
     const float* __restrict__ sub, const int32_t* __restrict__ tcnt, int K, T* __restrict__ dists,
@@ -563,7 +564,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   int64_t b, q0;
   const uint32_t item = static_cast<uint32_t>(lin0);
   const uint32_t way = ways == 1 ? 0u : item % static_cast<uint32_t>(ways);
-  const uint32_t lin = ways == 1 ? item : item / static_cast<uint32_t>(ways);
+  uint32_t lin = ways == 1 ? item : item / static_cast<uint32_t>(ways);
+  if (torder) lin = static_cast<uint32_t>(torder[lin]);  // the walk order of the query tiles
   if (qtiles) {
     const int64_t t = lin;
     int64_t lo = 0, hi = cl.B - 1;
@@ -1316,7 +1318,8 @@ static void launch_search(const PrunedLaunch& L, const PrunedState& S, dim3 grid
     }
   }
   hipLaunchKernelGGL(kern, grid, dim3(64), 0, st,
-                     static_cast<const T*>(L.q), L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.ptp,
+                     static_cast<const T*>(L.q), L.cl, L.cl.q_off ? S.qtp : nullptr, S.qperm, S.qkeys_s, S.torder,
+                     S.ptp,
                      S.stp, static_cast<const T*>(S.rec), S.hdr, S.shdr, S.sub, S.tcnt, static_cast<int>(L.K),
                      static_cast<T*>(L.dists), L.idxs, L.stats, ways, static_cast<T*>(S.lkey), S.lid,
                      static_cast<U*>(S.bnd), static_cast<const T*>(L.p), static_cast<const T*>(L.sigma),
@@ -1425,6 +1428,7 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
   S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
   S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.work = reinterpret_cast<unsigned*>(c.take(kXcds * 4));
+  S.torder = reinterpret_cast<int32_t*>(c.take(L.nqtiles * 4));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
   if (L.ways > 1) {
@@ -1531,7 +1535,65 @@ static hipError_t prep_points(const PrunedLaunch& L, const PrunedState& S, hipSt
   return hipGetLastError();
 }
 
-// Query side of every call: Morton codes in the cloud boxes, query order, tile prefix.
+// Walk order of the query tiles.  A tile's walk scans the union of its 64 queries'
+// neighbourhoods, so a tile whose queries spread over a wide region (its first and last
+// Morton codes differ in a high bit: the tile straddles far-apart regions of the sorted
+// order) takes several times the typical walk, and when such tiles start late they set
+// the kernel's end.  Within each XCD's contiguous range of tiles (the ranges of
+// xcd_block and of the persistent work lists), tiles are ordered by that bit, highest
+// first (longest walks first); equal bits in any order (the results do not depend on
+// the order).  One block per range.
+__device__ __forceinline__ int tile_spread(const uint64_t* __restrict__ keys, const Clouds& cl,
+                                           const int64_t* __restrict__ qtiles, int64_t t) {
+  int64_t q0, qe;
+  if (qtiles) {
+    int64_t lo = 0, hi = cl.B - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (qtiles[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    q0 = cl.q_begin(lo) + (t - qtiles[lo]) * 64;
+    qe = cl.q_end(lo);
+  } else {
+    const int64_t per = (cl.Q + 63) / 64;
+    const int64_t b = t / per;
+    q0 = b * cl.Q + (t - b * per) * 64;
+    qe = (b + 1) * cl.Q;
+  }
+  const int64_t q1 = q0 + 63 < qe ? q0 + 63 : qe - 1;
+  const uint64_t x = keys[q0] ^ keys[q1];
+  return x ? 64 - __builtin_clzll(x) : 0;  // 0..64
+}
+
+__global__ __launch_bounds__(1024) void tile_order_kernel(const uint64_t* __restrict__ keys, Clouds cl,
+                                                          const int64_t* __restrict__ qtiles, int64_t ntiles,
+                                                          int32_t* __restrict__ order) {
+  __shared__ unsigned cnt[65];
+  const int64_t qn = ntiles / kXcds, rn = ntiles % kXcds;
+  const int64_t x = blockIdx.x;
+  const int64_t lo = x * qn + (x < rn ? x : rn);
+  const int64_t hi = lo + qn + (x < rn ? 1 : 0);
+  for (int i = threadIdx.x; i < 65; i += blockDim.x) cnt[i] = 0u;
+  __syncthreads();
+  for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) atomicAdd(&cnt[tile_spread(keys, cl, qtiles, t)], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // bin starts, highest spread first
+    unsigned acc = 0;
+    for (int v = 64; v >= 0; --v) {
+      const unsigned c = cnt[v];
+      cnt[v] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+    const unsigned slot = atomicAdd(&cnt[tile_spread(keys, cl, qtiles, t)], 1u);
+    order[lo + slot] = static_cast<int32_t>(t);
+  }
+}
+
+// Query side of every call: Morton codes in the cloud boxes, query order, tile prefix,
+// tile walk order.
 template <typename T>
 static hipError_t prep_queries(const PrunedLaunch& L, const PrunedState& S, hipStream_t st) {
   const int bits = key_bits64(L.cl.B);
@@ -1545,6 +1607,9 @@ static hipError_t prep_queries(const PrunedLaunch& L, const PrunedState& S, hipS
   if (e != hipSuccess) return e;
   if (L.cl.q_off)
     hipLaunchKernelGGL(tile_prefix_kernel2, dim3(1), dim3(64), 0, st, L.cl.q_off, L.cl.B, int64_t(64), S.qtp);
+  if (L.nqtiles > 0)
+    hipLaunchKernelGGL(tile_order_kernel, dim3(kXcds), dim3(1024), 0, st, S.qkeys_s, L.cl,
+                       L.cl.q_off ? S.qtp : nullptr, L.nqtiles, S.torder);
   return hipGetLastError();
 }
 
