@@ -21,6 +21,7 @@ import json
 import math
 import os
 import platform
+import re
 import sys
 import time
 
@@ -81,7 +82,8 @@ def measured_traffic(workload):
     for line in open(path):
         line = line.strip()
         if line.startswith("library sources:"):
-            sha = line.split(":", 1)[1].strip().strip("`")
+            m = re.search(r"`([0-9a-f]+)`", line)
+            sha = m.group(1) if m else line.split(":", 1)[1].strip()
         if line.startswith("```traffic"):
             inside = True
         elif line.startswith("```"):
