@@ -54,9 +54,21 @@ struct PrunedState {
   int32_t* lid;     //                                    and indices
   void* bnd;        // split walk: shared per-query bound (ordered encoding) [qrows]
   unsigned* work;   // persistent search: next-item counters, one per XCD [kXcds]
-  int32_t* torder;  // query tiles in walk order (widest first within each XCD's range) [nqtiles]
+  int32_t* torder;  // walk items: (query tile, part) in walk order, widest first per XCD range [walk_items]
   size_t total;
 };
+
+// Split walks: per kSplitPer query tiles of each XCD's range, the kSplitTiles widest are
+// walked as kSplitParts parts of 64 / kSplitParts queries each (one wave per part; the
+// other lanes hold no query), so kSplitTiles * (kSplitParts - 1) extra walk items.  A wide tile's 64 queries straddle far-apart regions;
+// a part's 16 lie closer together, so each part walks a fraction of the tile's union
+// of neighbourhoods and the longest walks, which otherwise set the kernel's end, are
+// shortened.  Walk item codes: tile << 3 | part << 1 | split.
+constexpr int64_t kSplitPer = 128, kSplitTiles = 1, kSplitParts = 4;
+__host__ __device__ inline int64_t split_extra(int64_t ntiles) {  // extra items per XCD range
+  return ntiles < (int64_t(1) << 27) ? ntiles / (kSplitPer * kXcds) * kSplitTiles * (kSplitParts - 1) : 0;
+}
+__host__ __device__ inline int64_t walk_items(int64_t ntiles) { return ntiles + kXcds * split_extra(ntiles); }
 
 // Order-preserving unsigned encodings of the key (for atomic min on the shared bound).
 template <typename T>
@@ -565,7 +577,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   const uint32_t item = static_cast<uint32_t>(lin0);
   const uint32_t way = ways == 1 ? 0u : item % static_cast<uint32_t>(ways);
   uint32_t lin = ways == 1 ? item : item / static_cast<uint32_t>(ways);
-  if (torder) lin = static_cast<uint32_t>(torder[lin]);  // the walk order of the query tiles
+  int part = -1;  // a part of a split tile: its 64 / kSplitParts queries only
+  if (torder) {  // the walk order of the query tiles
+    const int32_t code = torder[lin];
+    lin = static_cast<uint32_t>(code) >> 3;
+    if (code & 1) part = (code >> 1) & 3;
+  }
   if (qtiles) {
     const int64_t t = lin;
     int64_t lo = 0, hi = cl.B - 1;
@@ -583,7 +600,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   }
   const int64_t qend = cl.q_end(b);
   const int64_t pos = q0 + lane;
-  const bool valid = pos < qend;
+  constexpr int kPartLanes = 64 / kSplitParts;
+  const bool valid = pos < qend && (part < 0 || lane / kPartLanes == part);
   int64_t row = 0;
   // lanes past the cloud's queries keep a finite dummy query and a bound of -inf (thr
   // below): they need no box and insert nothing
@@ -615,7 +633,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES ? WAVE
   }
   // seed: last tile whose first code <= the code of the wave's middle query (scalar
   // search): the walk then starts nearer to most lanes' own neighbourhoods
-  const uint32_t code0 = static_cast<uint32_t>(qkeys[min(q0 + 32, qend - 1)] & 0xffffffffu);
+  const uint32_t code0 =
+      static_cast<uint32_t>(qkeys[min(q0 + (part < 0 ? 32 : kPartLanes * part + kPartLanes / 2), qend - 1)] &
+                            0xffffffffu);
   int lo = tb, hi = te - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1335,9 +1355,8 @@ static hipError_t launch_pruned_k(const PrunedLaunch& L, const PrunedState& S, h
                        dim3(256), 0, st, static_cast<const T*>(L.q), L.cl, S.qperm, S.qkeys_s, S.ptp,
                        static_cast<const T*>(S.rec), S.hdr, static_cast<int>(L.K), L.qrows,
                        static_cast<U*>(S.bnd));
-  dim3 grid;
-  if (L.cl.q_off) grid = dim3(static_cast<unsigned>(L.nqtiles * ways), 1);
-  else grid = dim3(static_cast<unsigned>((L.cl.Q + 63) / 64 * ways), static_cast<unsigned>(L.cl.B));
+  // walk items (query tiles, and the parts of split ones) x ways, one dimension
+  const dim3 grid(static_cast<unsigned>(walk_items(L.nqtiles) * ways), 1);
   // tile geometry and occupancy variants (the host admits them for FP32, 9..16 neighbours)
   if constexpr (sizeof(T) == 4 && KMAX == 16) {
     if (L.sub_rec == 4) launch_search<T, KMAX, KEY, 4, 0>(L, S, grid, ways, st);
@@ -1428,7 +1447,7 @@ static PrunedState carve(const PrunedLaunch& L, size_t tmp_bytes) {
   S.qperm = reinterpret_cast<uint32_t*>(c.take(L.qrows * 4));
   S.qtp = reinterpret_cast<int64_t*>(c.take((L.cl.B + 1) * 8));
   S.work = reinterpret_cast<unsigned*>(c.take(kXcds * 4));
-  S.torder = reinterpret_cast<int32_t*>(c.take(L.nqtiles * 4));
+  S.torder = reinterpret_cast<int32_t*>(c.take(walk_items(L.nqtiles) * 4));
   S.tmp = c.take(tmp_bytes);
   S.tmp_bytes = tmp_bytes;
   if (L.ways > 1) {
@@ -1543,6 +1562,7 @@ static hipError_t prep_points(const PrunedLaunch& L, const PrunedState& S, hipSt
 // xcd_block and of the persistent work lists), tiles are ordered by that bit, highest
 // first (longest walks first); equal bits in any order (the results do not depend on
 // the order).  One block per range.
+
 __device__ __forceinline__ int tile_spread(const uint64_t* __restrict__ keys, const Clouds& cl,
                                            const int64_t* __restrict__ qtiles, int64_t t) {
   int64_t q0, qe;
@@ -1573,6 +1593,9 @@ __global__ __launch_bounds__(1024) void tile_order_kernel(const uint64_t* __rest
   const int64_t x = blockIdx.x;
   const int64_t lo = x * qn + (x < rn ? x : rn);
   const int64_t hi = lo + qn + (x < rn ? 1 : 0);
+  const int64_t extra = split_extra(ntiles);  // this range's extra items
+  const int64_t nsplit = extra / (kSplitParts - 1);
+  const int64_t ilo = lo + x * extra;         // this range's first walk item
   for (int i = threadIdx.x; i < 65; i += blockDim.x) cnt[i] = 0u;
   __syncthreads();
   for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) atomicAdd(&cnt[tile_spread(keys, cl, qtiles, t)], 1u);
@@ -1587,8 +1610,14 @@ __global__ __launch_bounds__(1024) void tile_order_kernel(const uint64_t* __rest
   }
   __syncthreads();
   for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-    const unsigned slot = atomicAdd(&cnt[tile_spread(keys, cl, qtiles, t)], 1u);
-    order[lo + slot] = static_cast<int32_t>(t);
+    const int64_t slot = atomicAdd(&cnt[tile_spread(keys, cl, qtiles, t)], 1u);
+    if (slot < nsplit) {  // one of the range's widest: in parts, first in the range
+#pragma unroll
+      for (int part = 0; part < kSplitParts; ++part)
+        order[ilo + kSplitParts * slot + part] = static_cast<int32_t>((t << 3) | (part << 1) | 1);
+    } else {
+      order[ilo + (kSplitParts - 1) * nsplit + slot] = static_cast<int32_t>(t << 3);
+    }
   }
 }
 
