@@ -233,6 +233,22 @@ def test_knn_wave_large_cloud(cuda, shape):
     assert torch.equal(idxs.cpu(), ri) and torch.equal(dists.cpu(), rd)
 
 
+@pytest.mark.parametrize("tune", [{"sub_records": 8, "super_tiles": 16, "mu_classes": 2},
+                                  {"sub_records": 4, "super_tiles": 8},
+                                  {"point_split": 1},
+                                  {"persistent": 1}])
+def test_knn_tiles_walk_order_and_parts_equal_brute(cuda, tune):
+    """131072 queries (2048 query tiles): the tiled search walks each XCD's tiles widest
+    first and the widest of every 128 as four 16-query parts; every row equals the brute
+    scan's (itself checked against the oracle above), in every launch mode."""
+    b = workloads.clustered_batch(Q=131072, P=16384, K=16, F=1, seed=21, spread=0.05)
+    d = b.to(cuda)
+    rd, ri = _C.knn_aniso(d.q, d.p, d.sigma, d.R, b.K, tune={"search": "brute"})
+    td, ti = _C.knn_aniso(d.q, d.p, d.sigma, d.R, b.K, tune=dict(tune, search="tiles"))
+    assert torch.equal(ti, ri)
+    assert torch.equal(td.view(torch.int32), rd.view(torch.int32))
+
+
 @pytest.mark.parametrize("K", [1, 5, 12, 40])
 @pytest.mark.parametrize("ways", [4, 64])
 def test_knn_tiles_split_walk_k_values(cuda, ways, K):
