@@ -155,6 +155,14 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
                             int64_t n, int begin_bit, int end_bit, void* ws, hipStream_t st,
                             const int32_t* vrank = nullptr, uint32_t vk = 1);
 
+// CSR transpose by two passes through packed words (aknn_sort.hip), for point keys of 9
+// to 16 bits whose entry numbers fit beside the high byte: off[0..rows] and the
+// entries' values (entry numbers, or vrank[q]*K + k), as the pair sort would give.
+bool csr_packed_fits(int64_t n, int64_t K, int key_bits);
+size_t csr_packed_workspace(int64_t n);
+hipError_t csr_sort_packed(const int64_t* idxs, const Clouds& cl, int64_t K, int64_t n, int64_t rows,
+                           const int32_t* vrank, int32_t* off, uint32_t* ent, void* ws, hipStream_t st);
+
 // CSR build
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, int64_t query_rows, size_t* bytes);
 // qorder (optional): entries number (position in qorder) * K + k instead of q * K + k

@@ -880,11 +880,17 @@ static int key_bits(int64_t rows) {
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Keys in and out and the pair sort's temporaries, or the packed sort's workspace.
+static size_t csr_sort_bytes(int64_t entries) {
+  const size_t pairs = 2 * align256(static_cast<size_t>(entries) * 4) + align256(radix_sort_workspace(entries, 4));
+  const size_t packed = align256(csr_packed_workspace(entries));
+  return pairs > packed ? pairs : packed;
+}
+
 hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, int64_t query_rows, size_t* bytes) {
   (void)point_rows;
-  // keys in and out, the sort's temporaries, the query ranks of an ordered transpose
-  *bytes = 2 * align256(static_cast<size_t>(entries) * 4) + align256(radix_sort_workspace(entries, 4)) +
-           align256(static_cast<size_t>(query_rows) * 4);
+  // the sort, then the query ranks of an ordered transpose
+  *bytes = csr_sort_bytes(entries) + align256(static_cast<size_t>(query_rows) * 4);
   return hipSuccess;
 }
 
@@ -897,6 +903,17 @@ hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, 
   uint32_t* keys_in = reinterpret_cast<uint32_t*>(base);
   uint32_t* keys_out = reinterpret_cast<uint32_t*>(base + align256(entries * 4));
   void* tmp = base + 2 * align256(entries * 4);
+  // in the spatial query order: the sort's values (entry numbers, which keep each row's
+  // entries in ascending q*K+k order) become position*K + k through the rank array
+  int32_t* rank = nullptr;
+  if (qorder && entries > 0) {
+    rank = reinterpret_cast<int32_t*>(base + csr_sort_bytes(entries));
+    hipLaunchKernelGGL(rank_kernel, dim3(static_cast<unsigned>((query_rows + 255) / 256)), dim3(256), 0, st,
+                       qorder, query_rows, rank);
+  }
+  if (csr_packed_fits(entries, K, key_bits(point_rows)) && reinterpret_cast<uintptr_t>(idxs) % 16 == 0)
+    return csr_sort_packed(idxs, cl, K, entries, point_rows, rank, csr_off, reinterpret_cast<uint32_t*>(csr_ent), ws,
+                           st);
   if (entries > 0) {
     const int bs = 256;
     if (K % 4 == 0 && reinterpret_cast<uintptr_t>(idxs) % 16 == 0) {
@@ -909,14 +926,6 @@ hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, 
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    // in the spatial query order: the sort's values (entry numbers, which keep each row's
-    // entries in ascending q*K+k order) become position*K + k through the rank array
-    int32_t* rank = nullptr;
-    if (qorder) {
-      rank = reinterpret_cast<int32_t*>(static_cast<char*>(tmp) + align256(radix_sort_workspace(entries, 4)));
-      hipLaunchKernelGGL(rank_kernel, dim3(static_cast<unsigned>((query_rows + 255) / 256)), dim3(256), 0, st,
-                         qorder, query_rows, rank);
-    }
     err = radix_sort_pairs<uint32_t>(keys_in, keys_out, nullptr, reinterpret_cast<uint32_t*>(csr_ent),
                                      entries, 0, key_bits(point_rows), tmp, st, rank, static_cast<uint32_t>(K));
     if (err != hipSuccess) return err;

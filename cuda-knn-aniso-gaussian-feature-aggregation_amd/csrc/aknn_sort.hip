@@ -129,15 +129,24 @@ __global__ __launch_bounds__(256) void digit_scan_kernel(uint32_t* __restrict__ 
 // The tile is ranked stably into LDS (digit-major, input order within a digit), then
 // written out in LDS order: consecutive threads write consecutive addresses of one
 // digit's run, so the global stores coalesce.
-template <typename K>
+//
+// kMode selects what goes out (32-bit keys only for modes 1 and 2):
+//   0  key and value pairs;
+//   1  one packed word per entry, ((key >> 8) << vbits) | value, to kout: the first of
+//      two passes over keys of at most 16 bits whose values fit below bit vbits;
+//   2  the value bits (word & (2^vbits - 1)) of packed words to vout: the second pass,
+//      whose digit is the word's top byte (shift = vbits).
+template <typename K, int kMode>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift, int64_t tiles,
     const uint32_t* __restrict__ start, const uint32_t* __restrict__ totals, const uint32_t* __restrict__ tile_counts,
-    K* __restrict__ kout, uint32_t* __restrict__ vout, const int32_t* __restrict__ vrank, uint32_t vk) {
+    K* __restrict__ kout, uint32_t* __restrict__ vout, const int32_t* __restrict__ vrank, uint32_t vk, int vbits) {
+  static_assert(kMode == 0 || sizeof(K) == 4, "packed passes take 32-bit words");
   constexpr int kSortItems = sort_items<K>();
   constexpr int kTileKeys = tile_keys<K>();
+  constexpr bool kVals = kMode != 2;  // mode 2 carries no separate value
   __shared__ K skey[kTileKeys];
-  __shared__ uint32_t sval[kTileKeys];
+  __shared__ uint32_t sval[kVals ? kTileKeys : 1];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
   __shared__ uint32_t gpos[256];       // global start of each digit's run of this tile
   __shared__ uint32_t wcnt[4][256];    // per-wave digit counts, then each wave's digit starts
@@ -171,11 +180,14 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const int jj = wbase + i * 64 + lane;
     const bool ok = jj < tile_n;
     kreg[i] = ok ? kin[base + jj] : K(0);
-    vreg[i] = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
-    if (!vin && vrank && ok) {  // position n = r*vk + k becomes vrank[r]*vk + k
-      const uint32_t n = vreg[i];
-      const uint32_t r = vdiv.div(n);
-      vreg[i] = static_cast<uint32_t>(vrank[r]) * vk + (n - r * vk);
+    vreg[i] = 0u;
+    if constexpr (kVals) {
+      vreg[i] = ok ? (vin ? vin[base + jj] : static_cast<uint32_t>(base + jj)) : 0u;
+      if (!vin && vrank && ok) {  // position n = r*vk + k becomes vrank[r]*vk + k
+        const uint32_t n = vreg[i];
+        const uint32_t r = vdiv.div(n);
+        vreg[i] = static_cast<uint32_t>(vrank[r]) * vk + (n - r * vk);
+      }
     }
   }
   __syncthreads();  // loc, gpos and the zeroed counts are visible
@@ -220,7 +232,7 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
       const uint32_t d = static_cast<uint32_t>((kreg[i] >> shift) & 255u);
       const uint32_t off = wcnt[wave][d] + lrank[i];
       skey[off] = kreg[i];
-      sval[off] = vreg[i];
+      if constexpr (kVals) sval[off] = vreg[i];
     }
   }
   __syncthreads();
@@ -229,8 +241,14 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K key = skey[jj];
     const uint32_t d = static_cast<uint32_t>((key >> shift) & 255u);
     const uint32_t pos = gpos[d] + (jj - loc[d]);
-    kout[pos] = key;
-    vout[pos] = sval[jj];
+    if constexpr (kMode == 0) {
+      kout[pos] = key;
+      vout[pos] = sval[jj];
+    } else if constexpr (kMode == 1) {
+      kout[pos] = ((static_cast<uint32_t>(key) >> 8) << vbits) | sval[jj];
+    } else {
+      vout[pos] = static_cast<uint32_t>(key) & ((1u << vbits) - 1u);
+    }
   }
 }
 
@@ -273,14 +291,145 @@ hipError_t radix_sort_pairs(const K* keys_in, K* keys_out, const uint32_t* vals_
     hipLaunchKernelGGL((digit_hist_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
                        ksrc, n, shift, tiles, counts, tile_counts);
     hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, st, counts, tiles, totals);
-    hipLaunchKernelGGL((digit_scatter_kernel<K>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
-                       ksrc, vsrc, n, shift, tiles, counts, totals, tile_counts, kdst, vdst, vrank, vk);
+    hipLaunchKernelGGL((digit_scatter_kernel<K, 0>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0, st,
+                       ksrc, vsrc, n, shift, tiles, counts, totals, tile_counts, kdst, vdst, vrank, vk, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ksrc = kdst;
     vsrc = vdst;
   }
   return hipSuccess;
+}
+
+// ---- CSR transpose of a neighbour table with point keys of 9 to 16 bits
+// The keys are the global point rows of the entries e = q*K + k; the sort's values the
+// entry numbers (or, with vrank, vrank[q]*K + k).  Two 8-bit passes, as above, with one
+// 32-bit word per entry in between: the first pass writes (key >> 8) << vbits | value,
+// the second takes its digit from the top byte and writes the values alone.  The row
+// starts come from the two passes' counts, without a search over sorted keys.
+
+// The first pass's histogram, with the keys made on the way: one 4096-entry tile per
+// block, thread t taking the 4-entry groups t, t + 256, ... (K divisible by 4, so a
+// group is one query row with one cloud lookup; idxs 16-byte aligned; n below 2^31).
+// The keys are written out and the counts stored as digit_hist stores them.
+__global__ __launch_bounds__(kSortThreads) void csr_keys_hist_kernel(const int64_t* __restrict__ idxs, Clouds cl,
+                                                                     uint32_t K, uint32_t n, uint32_t tiles,
+                                                                     uint32_t* __restrict__ keys,
+                                                                     uint32_t* __restrict__ counts,
+                                                                     uint32_t* __restrict__ tile_counts) {
+  constexpr int kTileKeys = tile_keys<uint32_t>();
+  constexpr int kGroups = kTileKeys / 4 / kSortThreads;
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const FastDiv kdiv(K);
+  const uint32_t base = blockIdx.x * kTileKeys;
+  uint4 kv[kGroups];
+#pragma unroll
+  for (int g = 0; g < kGroups; ++g) {
+    const uint32_t e = base + 4u * (g * kSortThreads + threadIdx.x);
+    kv[g] = make_uint4(0u, 0u, 0u, 0u);
+    if (e < n) {
+      const uint32_t pb = static_cast<uint32_t>(cl.p_begin(cl.cloud_of_query(kdiv.div(e))));
+      const longlong2 a = *reinterpret_cast<const longlong2*>(idxs + e);
+      const longlong2 c = *reinterpret_cast<const longlong2*>(idxs + e + 2);
+      kv[g] = make_uint4(pb + static_cast<uint32_t>(a.x), pb + static_cast<uint32_t>(a.y),
+                         pb + static_cast<uint32_t>(c.x), pb + static_cast<uint32_t>(c.y));
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < kGroups; ++g) {
+    const uint32_t e = base + 4u * (g * kSortThreads + threadIdx.x);
+    if (e < n) {
+      atomicAdd(&h[kv[g].x & 255u], 1u);
+      atomicAdd(&h[kv[g].y & 255u], 1u);
+      atomicAdd(&h[kv[g].z & 255u], 1u);
+      atomicAdd(&h[kv[g].w & 255u], 1u);
+      *reinterpret_cast<uint4*>(keys + e) = kv[g];
+    }
+  }
+  __syncthreads();
+  counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+  tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// off[k] = the number of entries with a key below k, for k = (hi << 8) | lo, block lo,
+// thread hi.  After the first pass the words are ordered by the low byte, so the
+// entries with a smaller key are those with a high byte below hi (the second pass's
+// digit totals) and those with high byte hi among the first X words, X = the number
+// of low bytes below lo: the second pass's scanned count for the tile holding word X,
+// plus the words with that high byte from the tile's start to X.
+__global__ __launch_bounds__(256) void csr_starts_kernel(const uint32_t* __restrict__ packed, int64_t n, int vbits,
+                                                         const uint32_t* __restrict__ totals1,
+                                                         const uint32_t* __restrict__ start2,
+                                                         const uint32_t* __restrict__ totals2, int64_t tiles,
+                                                         int64_t rows, int32_t* __restrict__ off) {
+  constexpr int kTileKeys = tile_keys<uint32_t>();
+  __shared__ uint32_t s[256], h[256];
+  const int lo = blockIdx.x, hi = threadIdx.x;
+  uint32_t ignored;
+  const uint32_t x = block_exscan256(s, hi < lo ? totals1[hi] : 0u, &ignored);
+  const int64_t tile = x / kTileKeys;
+  h[hi] = 0;
+  __syncthreads();
+  for (int64_t j = tile * kTileKeys + hi; j < x; j += 256) atomicAdd(&h[packed[j] >> vbits], 1u);
+  __syncthreads();
+  uint32_t below;
+  block_exscan256(s, totals2[hi], &below);
+  const uint32_t c = (tile < tiles ? start2[static_cast<int64_t>(hi) * tiles + tile] : totals2[hi]) + h[hi];
+  const int64_t k = (static_cast<int64_t>(hi) << 8) | lo;
+  if (k < rows) off[k] = static_cast<int32_t>(below + c);
+  if (lo == 0 && hi == 0) off[rows] = static_cast<int32_t>(n);
+}
+
+static int bits_for(int64_t count) {  // smallest b with 2^b >= count, at least 1
+  int b = 1;
+  while ((int64_t(1) << b) < count) ++b;
+  return b;
+}
+
+bool csr_packed_fits(int64_t n, int64_t K, int key_bits) {
+  return n > 0 && n < (int64_t(1) << 31) && K % 4 == 0 && key_bits > 8 && key_bits <= 16 &&
+         (key_bits - 8) + bits_for(n) <= 32;
+}
+
+size_t csr_packed_workspace(int64_t n) {
+  const int64_t tiles = (n + tile_keys<uint32_t>() - 1) / tile_keys<uint32_t>();
+  return 2 * al(n * 4) + 2 * al(tiles * 256 * 4) + 2 * al(256 * 4);
+}
+
+hipError_t csr_sort_packed(const int64_t* idxs, const Clouds& cl, int64_t K, int64_t n, int64_t rows,
+                           const int32_t* vrank, int32_t* off, uint32_t* ent, void* ws, hipStream_t st) {
+  constexpr int64_t kTileKeys = tile_keys<uint32_t>();
+  const int64_t tiles = (n + kTileKeys - 1) / kTileKeys;
+  const int64_t m = tiles * 256;
+  const int vbits = bits_for(n);
+  char* w = static_cast<char*>(ws);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(w);
+  w += al(n * 4);
+  uint32_t* packed = reinterpret_cast<uint32_t*>(w);
+  w += al(n * 4);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(w);
+  w += al(m * 4);
+  uint32_t* tile_counts = reinterpret_cast<uint32_t*>(w);
+  w += al(m * 4);
+  uint32_t* totals1 = reinterpret_cast<uint32_t*>(w);
+  w += al(256 * 4);
+  uint32_t* totals2 = reinterpret_cast<uint32_t*>(w);
+  const dim3 grid(static_cast<unsigned>(tiles)), block(kSortThreads);
+  hipLaunchKernelGGL(csr_keys_hist_kernel, grid, block, 0, st, idxs, cl, static_cast<uint32_t>(K),
+                     static_cast<uint32_t>(n), static_cast<uint32_t>(tiles), keys, counts, tile_counts);
+  hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, st, counts, tiles, totals1);
+  hipLaunchKernelGGL((digit_scatter_kernel<uint32_t, 1>), grid, block, 0, st, keys, nullptr, n, 0, tiles, counts,
+                     totals1, tile_counts, packed, nullptr, vrank, static_cast<uint32_t>(K), vbits);
+  hipLaunchKernelGGL((digit_hist_kernel<uint32_t>), grid, block, 0, st, packed, n, vbits, tiles, counts,
+                     tile_counts);
+  hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, st, counts, tiles, totals2);
+  hipLaunchKernelGGL(csr_starts_kernel, dim3(256), dim3(256), 0, st, packed, n, vbits, totals1, counts, totals2,
+                     tiles, rows, off);
+  hipLaunchKernelGGL((digit_scatter_kernel<uint32_t, 2>), grid, block, 0, st, packed, nullptr, n, vbits, tiles,
+                     counts, totals2, tile_counts, nullptr, ent, nullptr, 0u, vbits);
+  return hipGetLastError();
 }
 
 template hipError_t radix_sort_pairs<uint32_t>(const uint32_t*, uint32_t*, const uint32_t*, uint32_t*, int64_t,

@@ -509,21 +509,50 @@ def test_cache_does_not_change_results(cuda, search):
         assert torch.equal(x, y)
 
 
-@pytest.mark.parametrize("P,Q,K", [(300, 700, 8), (65536, 20000, 16), (5000, 300000, 16)])
-def test_csr_transpose(cuda, P, Q, K):
+def _csr_expected(idxs, P):
+    """Offsets and entries of the transpose, from torch's stable CPU sort of the global
+    point rows (cloud b's rows start at b * P)."""
+    B, Q, K = idxs.shape
+    keys = (idxs + torch.arange(B).view(B, 1, 1) * P).reshape(-1)
+    order = torch.sort(keys, stable=True).indices
+    counts = torch.bincount(keys, minlength=B * P)
+    off = torch.zeros(B * P + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(counts, 0)
+    return off, order
+
+
+# Point keys of 9 to 16 bits with K divisible by 4 take the two packed passes (the
+# 1M x 16 x 64K case fills all 32 bits of the packed word); 8-bit keys (one pass),
+# 17-bit keys and K = 3 take the key and value pair sort.
+@pytest.mark.parametrize("B,P,Q,K", [(1, 300, 700, 8), (1, 65536, 20000, 16), (1, 5000, 300000, 16),
+                                     (3, 3000, 7000, 12), (1, 200, 1000, 4), (1, 70000, 5000, 16),
+                                     (2, 5000, 3000, 3), (1, 65536, 1 << 20, 16)])
+def test_csr_transpose(cuda, B, P, Q, K):
     """The point-side transpose of idxs: every entry once, each row's entries ascending
     (the fixed reduction order), offsets = exclusive scan of the per-point counts."""
-    g = torch.Generator().manual_seed(P)
-    idxs = torch.randint(P, (1, Q, K), generator=g)
+    g = torch.Generator().manual_seed(P + Q)
+    idxs = torch.randint(P, (B, Q, K), generator=g)
     idxs[0, : Q // 3, 0] = 7  # one heavy row
     off, ent = _C.build_csr(idxs.to(cuda), P)
-    off, ent = off.cpu().long(), ent.cpu().long()
-    counts = torch.bincount(idxs.reshape(-1), minlength=P)
-    assert torch.equal(off[1:] - off[:-1], counts) and off[0] == 0
-    key = idxs.reshape(-1)[ent]                      # point row of each placed entry
-    assert torch.equal(key, torch.sort(idxs.reshape(-1), stable=True).values)
-    ref = torch.sort(idxs.reshape(-1), stable=True).indices
-    assert torch.equal(ent, ref)
+    off_ref, ent_ref = _csr_expected(idxs, P)
+    assert torch.equal(off.cpu().long(), off_ref)
+    assert torch.equal(ent.cpu().long(), ent_ref)
+
+
+@pytest.mark.parametrize("B,P,Q,K", [(1, 4000, 50000, 16), (2, 300, 2000, 8), (1, 200, 3000, 5)])
+def test_csr_transpose_in_query_order(cuda, B, P, Q, K):
+    """With a query order, entry q*K + k is numbered rank(q)*K + k (rank = the query's
+    position in the order) and each row keeps the ascending q*K + k order."""
+    g = torch.Generator().manual_seed(Q)
+    idxs = torch.randint(P, (B, Q, K), generator=g)
+    order = torch.randperm(B * Q, generator=g).to(torch.int32)
+    off, ent, _ = _C.build_csr(idxs.to(cuda), P, order=order.to(cuda))
+    off_ref, ent_ref = _csr_expected(idxs, P)
+    rank = torch.empty(B * Q, dtype=torch.int64)
+    rank[order.long()] = torch.arange(B * Q)
+    numbered = rank[ent_ref // K] * K + ent_ref % K
+    assert torch.equal(off.cpu().long(), off_ref)
+    assert torch.equal(ent.cpu().long(), numbered)
 
 
 @pytest.mark.parametrize("key", ["aniso", "euclidean"])
