@@ -145,24 +145,30 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   constexpr int kSortItems = sort_items<K>();
   constexpr int kTileKeys = tile_keys<K>();
   constexpr bool kVals = kMode != 2;  // mode 2 carries no separate value
+  // mode 1 places the packed word, and beside it the digit it no longer holds
   __shared__ K skey[kTileKeys];
-  __shared__ uint32_t sval[kVals ? kTileKeys : 1];
+  __shared__ uint32_t sval[kMode == 0 ? kTileKeys : 1];
+  __shared__ uint8_t sdig[kMode == 1 ? kTileKeys : 1];
   __shared__ uint32_t loc[256];        // local start of each digit in the tile
   __shared__ uint32_t gpos[256];       // global start of each digit's run of this tile
   __shared__ uint32_t wcnt[4][256];    // per-wave digit counts, then each wave's digit starts
   __shared__ uint32_t scan_tmp[256];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kTileKeys;
+  // Each XCD takes one contiguous range of tiles: the runs that adjacent tiles write for
+  // one digit are adjacent in the output, and the line at their junction is then
+  // completed in one L2.
+  const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t base = tile * kTileKeys;
   const int tile_n = static_cast<int>(min<int64_t>(kTileKeys, n - base));
 
   // local digit starts from the tile's histogram (digit_hist), global starts from the scan
   uint32_t ex;
-  block_exscan256(scan_tmp, tile_counts[static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x], &ex);
+  block_exscan256(scan_tmp, tile_counts[tile * 256 + threadIdx.x], &ex);
   loc[threadIdx.x] = ex;
   uint32_t dbase;  // this digit's first output position: the counts of the lower digits
   block_exscan256(scan_tmp, totals[threadIdx.x], &dbase);
-  gpos[threadIdx.x] = dbase + start[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
+  gpos[threadIdx.x] = dbase + start[static_cast<int64_t>(threadIdx.x) * tiles + tile];
 #pragma unroll
   for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
 
@@ -231,21 +237,26 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     if (wbase + i * 64 + lane < tile_n) {
       const uint32_t d = static_cast<uint32_t>((kreg[i] >> shift) & 255u);
       const uint32_t off = wcnt[wave][d] + lrank[i];
-      skey[off] = kreg[i];
-      if constexpr (kVals) sval[off] = vreg[i];
+      if constexpr (kMode == 1) {
+        skey[off] = ((static_cast<uint32_t>(kreg[i]) >> 8) << vbits) | vreg[i];
+        sdig[off] = static_cast<uint8_t>(d);
+      } else {
+        skey[off] = kreg[i];
+        if constexpr (kVals) sval[off] = vreg[i];
+      }
     }
   }
   __syncthreads();
   // coalesced write-out of the locally sorted tile
   for (int jj = threadIdx.x; jj < tile_n; jj += kSortThreads) {
     const K key = skey[jj];
-    const uint32_t d = static_cast<uint32_t>((key >> shift) & 255u);
+    const uint32_t d = kMode == 1 ? sdig[jj] : static_cast<uint32_t>((key >> shift) & 255u);
     const uint32_t pos = gpos[d] + (jj - loc[d]);
     if constexpr (kMode == 0) {
       kout[pos] = key;
       vout[pos] = sval[jj];
     } else if constexpr (kMode == 1) {
-      kout[pos] = ((static_cast<uint32_t>(key) >> 8) << vbits) | sval[jj];
+      kout[pos] = key;
     } else {
       vout[pos] = static_cast<uint32_t>(key) & ((1u << vbits) - 1u);
     }
