@@ -526,7 +526,7 @@ def _csr_expected(idxs, P):
 # 17-bit keys and K = 3 take the key and value pair sort.
 @pytest.mark.parametrize("B,P,Q,K", [(1, 300, 700, 8), (1, 65536, 20000, 16), (1, 5000, 300000, 16),
                                      (3, 3000, 7000, 12), (1, 200, 1000, 4), (1, 70000, 5000, 16),
-                                     (2, 5000, 3000, 3), (1, 65536, 1 << 20, 16)])
+                                     (1, 70000, 40000, 16), (2, 5000, 3000, 3), (1, 65536, 1 << 20, 16)])
 def test_csr_transpose(cuda, B, P, Q, K):
     """The point-side transpose of idxs: every entry once, each row's entries ascending
     (the fixed reduction order), offsets = exclusive scan of the per-point counts."""
