@@ -466,24 +466,14 @@ static int forward_run(const aknn_dims* d, const aknn_tune* tune, const void* q,
   const void* rec = nullptr;
   const int32_t* order = reinterpret_cast<const int32_t*>(qord);
   if (cache) {
-    hipError_t e = hipSuccess;
-    if (cache->query_order) {
-      e = qord ? aknn::launch_copy_u32(qord, cache->query_order, query_rows(d), st)
-               : aknn::launch_identity(cache->query_order, query_rows(d), st);
-      if (e != hipSuccess) return hip_fail(e, "cache");
-    }
-    if (cache->point_order) {
-      e = pord ? aknn::launch_copy_u32(pord, cache->point_order, point_rows(d), st)
-               : aknn::launch_identity(cache->point_order, point_rows(d), st);
-      if (e != hipSuccess) return hip_fail(e, "cache");
-    }
-    if (cache->point_records) {
-      e = d->dtype == AKNN_F32
-              ? aknn::launch_pack_points<float>(p, sigma, R, cache->point_records, point_rows(d), st)
-              : aknn::launch_pack_points<double>(p, sigma, R, cache->point_records, point_rows(d), st);
-      if (e != hipSuccess) return hip_fail(e, "cache");
-      rec = cache->point_records;
-    }
+    const hipError_t e =
+        d->dtype == AKNN_F32
+            ? aknn::launch_cache_fill<float>(qord, cache->query_order, query_rows(d), pord, cache->point_order,
+                                             point_rows(d), p, sigma, R, cache->point_records, st)
+            : aknn::launch_cache_fill<double>(qord, cache->query_order, query_rows(d), pord, cache->point_order,
+                                              point_rows(d), p, sigma, R, cache->point_records, st);
+    if (e != hipSuccess) return hip_fail(e, "cache");
+    rec = cache->point_records;
   }
   aknn::QueryLaunch L{};
   L.dtype = d->dtype;

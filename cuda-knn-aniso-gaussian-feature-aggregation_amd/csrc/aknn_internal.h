@@ -133,8 +133,12 @@ hipError_t launch_point_dots(const PointLaunch& L, void* a, hipStream_t st);
 // order as the per-entry shares are
 hipError_t launch_dq_dots(const QueryLaunch& L, bool second_order, hipStream_t st);
 
-hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st);
-hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st);
+// The forward's cache (query order, point order, point records; each may be null) in one
+// launch; a null order is written as the identity (aknn_knn.hip).
+template <typename T>
+hipError_t launch_cache_fill(const uint32_t* qord, int32_t* qout, int64_t qrows, const uint32_t* pord,
+                             int32_t* pout, int64_t prows, const void* p, const void* sigma, const void* R,
+                             void* rec, hipStream_t st);
 hipError_t pruned_workspace(const PrunedLaunch& L, size_t* bytes);
 size_t pruned_index_bytes(const PrunedLaunch& L);
 size_t pruned_index_build_ws(const PrunedLaunch& L);

@@ -1713,26 +1713,4 @@ hipError_t launch_pruned_index_build(const PrunedLaunch& Lin, hipStream_t st) {
 const uint32_t* pruned_query_order(const PrunedLaunch& L) { return L.wave ? nullptr : carve_run(L).qperm; }
 const uint32_t* pruned_point_order(const PrunedLaunch& L) { return carve_run(L).pperm; }
 
-__global__ void identity_kernel(int32_t* __restrict__ out, int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = static_cast<int32_t>(i);
-}
-
-__global__ void copy_u32_kernel(const uint32_t* __restrict__ src, int32_t* __restrict__ dst, int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = static_cast<int32_t>(src[i]);
-}
-
-hipError_t launch_identity(int32_t* out, int64_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(identity_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, out, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_copy_u32(const uint32_t* src, int32_t* dst, int64_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(copy_u32_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
-  return hipGetLastError();
-}
-
 }  // namespace aknn
