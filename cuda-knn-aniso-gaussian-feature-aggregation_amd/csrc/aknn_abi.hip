@@ -1061,8 +1061,8 @@ static int backward_common(const aknn_dims* d, uint32_t needs, bool second, cons
                     total);
       q_sorted = static_cast<char*>(ws) + base;
       hh_sorted = static_cast<char*>(q_sorted) + ordered_copy_bytes(d);
-      hipError_t e = aknn::launch_gather3(q, cache->query_order, q_sorted, query_rows(d), d->dtype, st, gg,
-                                          gg ? hh_sorted : nullptr);
+      hipError_t e = aknn::launch_gather3(q, cache->query_order, q_sorted, query_rows(d), d->dtype, st);
+      if (e == hipSuccess && gg) e = aknn::launch_gather3(gg, cache->query_order, hh_sorted, query_rows(d), d->dtype, st);
       if (e != hipSuccess) return hip_fail(e, "query_order copies");
     }
     aknn::PointLaunch L{};

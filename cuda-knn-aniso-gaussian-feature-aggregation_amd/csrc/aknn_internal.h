@@ -174,9 +174,8 @@ hipError_t csr_temp_bytes(int64_t entries, int64_t point_rows, int64_t query_row
 hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, int64_t K,
                             int64_t point_rows, int32_t* csr_off, int32_t* csr_ent, void* ws,
                             size_t ws_bytes, hipStream_t st, const int32_t* qorder = nullptr);
-// out[i][0..2] = src[order[i]][0..2] (query-side rows in spatial order); with src2, the
-// same for src2 into out2 in the same launch
+// out[i][0..2] = src[order[i]][0..2] (query-side rows in spatial order)
 hipError_t launch_gather3(const void* src, const int32_t* order, void* out, int64_t rows, int32_t dtype,
-                          hipStream_t st, const void* src2 = nullptr, void* out2 = nullptr);
+                          hipStream_t st);
 
 }  // namespace aknn

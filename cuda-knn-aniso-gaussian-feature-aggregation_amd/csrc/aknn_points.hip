@@ -936,34 +936,27 @@ hipError_t launch_build_csr(const int64_t* idxs, Clouds cl, int64_t query_rows, 
   return hipGetLastError();
 }
 
-// One or two arrays of 3-vectors in the same order (src2 / out2 may be null).
 template <typename T>
 __global__ void gather3_kernel(const T* __restrict__ src, const int32_t* __restrict__ order, T* __restrict__ out,
-                               int64_t rows, const T* __restrict__ src2, T* __restrict__ out2) {
+                               int64_t rows) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= rows) return;
   const int64_t r = order[i];
   out[i * 3 + 0] = src[r * 3 + 0];
   out[i * 3 + 1] = src[r * 3 + 1];
   out[i * 3 + 2] = src[r * 3 + 2];
-  if (src2) {
-    out2[i * 3 + 0] = src2[r * 3 + 0];
-    out2[i * 3 + 1] = src2[r * 3 + 1];
-    out2[i * 3 + 2] = src2[r * 3 + 2];
-  }
 }
 
 hipError_t launch_gather3(const void* src, const int32_t* order, void* out, int64_t rows, int32_t dtype,
-                          hipStream_t st, const void* src2, void* out2) {
+                          hipStream_t st) {
   if (rows <= 0) return hipSuccess;
   const dim3 grid(static_cast<unsigned>((rows + 255) / 256)), block(256);
   if (dtype == AKNN_F32)
     hipLaunchKernelGGL((gather3_kernel<float>), grid, block, 0, st, static_cast<const float*>(src), order,
-                       static_cast<float*>(out), rows, static_cast<const float*>(src2), static_cast<float*>(out2));
+                       static_cast<float*>(out), rows);
   else
     hipLaunchKernelGGL((gather3_kernel<double>), grid, block, 0, st, static_cast<const double*>(src), order,
-                       static_cast<double*>(out), rows, static_cast<const double*>(src2),
-                       static_cast<double*>(out2));
+                       static_cast<double*>(out), rows);
   return hipGetLastError();
 }
 
